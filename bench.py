@@ -1,0 +1,200 @@
+"""Benchmark: W-projection gridding throughput on one MI355X.
+
+Workload (default): the 31,395,840-visibility / 601-plane / support-72 /
+16384^2 synthetic shape of the reference's SKA-LOW EL56-82 data set
+(README:126-134), FP32, inputs resident in HBM.  A step is one pass of the
+gridding pipeline over all visibilities: kernel-table expansion, selection,
+bucketing by tile, tile gridding and the ordered merge, accumulating into
+the device grid (the drop-in's semantics).
+
+Prints one JSON line.  `roofline` prices the tile-gridding kernel: its
+algorithmic FP32 work, 8 flops per complex tap MAC summed over gridded
+visibilities' (2s+1)^2 taps, divided by the kernel's average duration from
+HIP events recorded on the launch stream inside the timed steps.
+`cpu_baseline` times the reference's serial oskar_grid_wproj_f (compiled
+from its sources into oracle/_ref, else the oracle's restatement) on one
+core on a prefix of the same visibilities.
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "gpu-gridding_amd"))
+
+PEAK_FP32_TFLOPS = 157.3          # MI355X vector/matrix FP32, packed FMA
+PEAK_FP32_NONPACKED_TFLOPS = 78.6  # one FMA per lane per clock
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(case, seconds):
+    """Serial gridder on one core over a prefix of the workload."""
+    import numpy as np
+    from oracle import oracle
+    use_ref = oracle.ref() is not None
+    kind = "reference" if use_ref else "port"
+    G = case["grid_size"]
+    grid = np.zeros((G, G, 2), np.float32)
+
+    def run(n):
+        sub = dict(case)
+        for k in ("uu", "vv", "ww", "weight"):
+            sub[k] = case[k][:n]
+        sub["vis"] = case["vis"][:n]
+        sub["num_points"] = n
+        t0 = time.perf_counter()
+        oracle.grid_f(sub, use_ref=use_ref, grid=grid)
+        return time.perf_counter() - t0
+
+    n0 = 20_000
+    t0 = run(n0)
+    n = int(min(case["num_points"], max(n0, n0 * seconds / max(t0, 1e-6))))
+    t = run(n)
+    return dict(value=n / t / 1e6, unit="Mvis/s", cores=1, kind=kind,
+                sample="first %d of %d visibilities (%.1f s), serial oskar_grid_wproj_f, "
+                       "%s" % (n, case["num_points"], t, platform.processor() or "x86-64"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--compensated", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU only (the gridder has no multi-GPU "
+              "path); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from mi355_gridding import synth, _lib, torch_ops as ops
+
+    torch.cuda.set_device(0)
+    t0 = time.time()
+    case = synth.make(a.config)
+    log("generated %s: N=%d G=%d P=%d smax=%d in %.1fs" % (
+        case["name"], case["num_points"], case["grid_size"], case["num_w_planes"],
+        case["max_support"], time.time() - t0))
+    N, G = case["num_points"], case["grid_size"]
+    ks = ops.KernelStack(case["support"], case["oversample"], case["conv_size_half"],
+                         case["conv_func"])
+    d = {k: torch.from_numpy(np.ascontiguousarray(case[k])).cuda()
+         for k in ("uu", "vv", "ww", "vis", "weight")}
+    grid = torch.zeros((G, G, 2), dtype=torch.float32, device="cuda")
+    norm = torch.zeros(1, dtype=torch.float64, device="cuda")
+    skipped = torch.zeros(1, dtype=torch.int64, device="cuda")
+    ctx = ops.context()
+    ctx.set_option(_lib.OPT_PHASE_TIMING, 1)
+    ctx.set_option(_lib.OPT_COMPENSATED, 1 if a.compensated else 0)
+
+    def step():
+        ops.grid(ks, d["uu"], d["vv"], d["ww"], d["vis"], d["weight"], case["cell_size_rad"],
+                 case["w_scale"], G, grid, norm, skipped, ctx=ctx)
+
+    for i in range(a.warmup):
+        step()
+        log("warmup %d done" % i)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    phases = []
+    w0 = time.perf_counter()
+    ev0.record(stream)
+    for i in range(a.steps):
+        step()
+        phases.append(ctx.stats())
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ms = ev0.elapsed_time(ev1) / a.steps
+    log("timed %d steps: %.3f ms/step (wall %.3f ms/step)" % (a.steps, ms, wall * 1e3 / a.steps))
+    st = phases[-1]
+    grid_ms = float(np.mean([p["ms_grid"] for p in phases]))
+    flops = 8.0 * st["num_taps"]
+    achieved = flops / (grid_ms * 1e-3) / 1e12
+    phase_ms = {k: round(float(np.mean([p[k] for p in phases])), 4)
+                for k in ("ms_prepare", "ms_select", "ms_bucket", "ms_grid", "ms_merge",
+                          "ms_total")}
+
+    accuracy = None
+    if not a.no_accuracy:
+        # FP32 grid of one fresh pass vs the FP64 truth with the same selection.
+        g32 = torch.zeros_like(grid)
+        n32 = torch.zeros_like(norm)
+        step_args = (ks, d["uu"], d["vv"], d["ww"], d["vis"], d["weight"], case["cell_size_rad"],
+                     case["w_scale"], G)
+        ops.grid(*step_args, g32, n32, skipped, ctx=ctx)
+        g64 = torch.zeros((G, G, 2), dtype=torch.float64, device="cuda")
+        n64 = torch.zeros_like(norm)
+        ops.grid_fsel_d(*step_args, g64, n64, skipped, ctx=ctx)
+        ctx.set_option(_lib.OPT_COMPENSATED, 1)
+        gc = torch.zeros_like(grid)
+        ops.grid(*step_args, gc, torch.zeros_like(norm), skipped, ctx=ctx)
+        ctx.set_option(_lib.OPT_COMPENSATED, 1 if a.compensated else 0)
+        torch.cuda.synchronize()
+        m64 = g64.abs().max().item()
+        l64 = torch.linalg.norm(g64).item()
+
+        def err(g):
+            dd = g.double() - g64
+            return dict(max_rel=dd.abs().max().item() / m64,
+                        l2_rel=torch.linalg.norm(dd).item() / l64)
+        accuracy = dict(plain_fp32=err(g32), compensated_fp32=err(gc),
+                        norm_rel=abs(n32.item() - n64.item()) / abs(n64.item()),
+                        truth="FP64 accumulation, FP32 selection (mi355_grid_wproj_fsel_d_dev)")
+        del g64, gc, g32
+        log("accuracy %s" % json.dumps(accuracy))
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(case, a.cpu_seconds)
+        log("cpu baseline %s" % json.dumps(cpu))
+
+    out = {
+        "metric": "Mvis/s and % FP32 peak, 31.4M-vis/601-plane grid, at <=1e-5 rel err vs FP64",
+        "value": round(N / (ms * 1e-3) / 1e6, 3),
+        "unit": "Mvis/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, shapes of the reference data set)",
+        "config": {"workload": case["name"], "num_points": N, "grid_size": G,
+                   "num_w_planes": case["num_w_planes"], "max_support": case["max_support"],
+                   "oversample": case["oversample"],
+                   "accumulation": "compensated" if a.compensated else "plain"},
+        "roofline": {"bound": "fp32-valu", "achieved": round(achieved, 3),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "frac_nonpacked": round(achieved / PEAK_FP32_NONPACKED_TFLOPS, 4),
+                     "kernel": "k_grid_tiles", "kernel_ms": round(grid_ms, 4),
+                     "flops_per_launch": flops, "traffic": None},
+        "phases_ms": phase_ms,
+        "work": {"num_taps": st["num_taps"], "num_visits": st["num_visits"],
+                 "num_items": st["num_items"], "num_skipped": st["num_skipped"]},
+        "accuracy": accuracy,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1085 @@
+// gridder.hip -- host orchestration and C ABI of the MI355X W-projection gridder.
+//
+// Replaces oskar_grid_wproj_f/_d (oskar_grid_wproj.h:68-131) behind the same
+// signature.  One call = one pass of the pipeline in gg_kernels.hpp over all
+// of the call's visibilities, on the context's stream, with two small
+// device-to-host reads (visit total, work-item total) to size buffers.
+#include "gg_kernels.hpp"
+#include "mi355_gridder.h"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace gg;
+
+// ---------------------------------------------------------------------------
+// Status
+// ---------------------------------------------------------------------------
+namespace {
+
+thread_local int t_last_status = MI355_OK;
+thread_local char t_last_msg[512] = "";
+
+int fail(int status, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(t_last_msg, sizeof(t_last_msg), fmt, ap);
+    va_end(ap);
+    t_last_status = status;
+    return status;
+}
+
+int ok()
+{
+    t_last_status = MI355_OK;
+    t_last_msg[0] = 0;
+    return MI355_OK;
+}
+
+#define HIP_TRY(expr)                                                        \
+    do {                                                                     \
+        hipError_t e_ = (expr);                                              \
+        if (e_ != hipSuccess)                                                \
+            return fail(e_ == hipErrorOutOfMemory ? MI355_ERR_OUT_OF_MEMORY  \
+                                                  : MI355_ERR_HIP,           \
+                    "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),   \
+                    __FILE__, __LINE__);                                     \
+    } while (0)
+
+#define TRY(expr)                                                            \
+    do { int s_ = (expr); if (s_ != MI355_OK) return s_; } while (0)
+
+// Grow-only device allocation.
+struct DevBuf
+{
+    void* p = nullptr;
+    size_t cap = 0;
+    template <typename X> X* as() const { return static_cast<X*>(p); }
+};
+
+int ensure(DevBuf& b, size_t bytes)
+{
+    if (bytes == 0) bytes = 256;
+    if (bytes <= b.cap) return MI355_OK;
+    if (b.p)
+    {
+        // Work already queued may still read the old buffer.
+        (void)hipDeviceSynchronize();
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    const size_t want = bytes + bytes / 8;      // headroom for slowly growing calls
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess)
+    {
+        (void)hipGetLastError();
+        e = hipMalloc(&b.p, bytes);
+        if (e != hipSuccess)
+        {
+            (void)hipGetLastError();
+            b.p = nullptr;
+            return fail(MI355_ERR_OUT_OF_MEMORY, "device allocation of %zu bytes failed", bytes);
+        }
+        b.cap = bytes;
+        return MI355_OK;
+    }
+    b.cap = want;
+    return MI355_OK;
+}
+
+void release(DevBuf& b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+int bits_for(unsigned long long n)   // bits needed to hold values 0..n-1
+{
+    int b = 0;
+    while (b < 64 && (1ull << b) < n) ++b;
+    return b;
+}
+
+unsigned grid_blocks(unsigned long long work, unsigned per_block, unsigned cap)
+{
+    unsigned long long b = (work + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned)b;
+}
+
+enum Phase { EV_START, EV_PREP, EV_SELECT, EV_BUCKET, EV_GRID, EV_MERGE, EV_COUNT };
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct mi355_ctx
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    unsigned visits_per_item = 2048;
+    bool compensated = false;
+    bool timing = false;
+    bool cache_kernels = false;
+
+    // prepared kernel tables
+    const void* cache_conv = nullptr;
+    int cache_prec = 0, cache_ov = 0, cache_csh = 0;
+    std::vector<int> cache_support;
+    bool tables_valid = false;
+    std::vector<unsigned> plane_off;
+    unsigned long long table_elems = 0;
+    int max_support = 0;
+
+    DevBuf tables, norm_table, support_d, plane_off_d;
+    DevBuf meta, amp, num_visits, visit_off;
+    DevBuf part_norm, part_skipped, part_taps, part_bbox;
+    DevBuf scan_partial, scalars;
+    DevBuf keys[2], vals[2], hist, hist_off;
+    DevBuf tile_start, tile_end, n_items, n_slots, item_off, slot_off, item_tile;
+    DevBuf scratch;
+    DevBuf jobs_d, job_chunk_off_d, job_vis_off_d, job_tile_base_d, job_out_d, job_bbox_d;
+    // drop-in staging
+    DevBuf in_conv, in_uu, in_vv, in_ww, in_vis, in_weight, in_grid, in_norm, in_skipped;
+
+    // host mirrors kept alive across asynchronous uploads
+    std::vector<unsigned char> h_jobs;
+    std::vector<unsigned> h_job_chunk_off, h_job_tile_base;
+    std::vector<unsigned long long> h_job_vis_off;
+    std::vector<unsigned long long> h_job_out;
+    std::vector<int4> h_job_bbox;
+
+    hipEvent_t ev[EV_COUNT] = {};
+    mi355_stats stats = {};
+};
+
+namespace {
+
+std::mutex g_default_lock;
+mi355_ctx* g_default[64] = {};
+
+void record(mi355_ctx* C, Phase p)
+{
+    if (C->timing) (void)hipEventRecord(C->ev[p], C->stream);
+}
+
+int init_ctx(mi355_ctx* C, int device)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+    {
+        (void)hipGetLastError();
+        return fail(MI355_ERR_NO_DEVICE, "no HIP device visible");
+    }
+    if (device < 0 || device >= n)
+        return fail(MI355_ERR_INVALID_ARG, "device %d out of range (have %d)", device, n);
+    C->device = device;
+    HIP_TRY(hipSetDevice(device));
+    for (int i = 0; i < EV_COUNT; ++i) HIP_TRY(hipEventCreate(&C->ev[i]));
+    return MI355_OK;
+}
+
+int get_default_ctx(mi355_ctx** out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return fail(MI355_ERR_NO_DEVICE, "no HIP device visible");
+    }
+    if (dev < 0 || dev >= 64) return fail(MI355_ERR_INVALID_ARG, "device id %d", dev);
+    if (!g_default[dev])
+    {
+        mi355_ctx* C = new mi355_ctx();
+        const int s = init_ctx(C, dev);
+        if (s != MI355_OK) { delete C; return s; }
+        g_default[dev] = C;
+    }
+    *out = g_default[dev];
+    return MI355_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Argument checks shared by every entry point (host only, before any HIP call)
+// ---------------------------------------------------------------------------
+int check_kernel_args(size_t num_w_planes, const int* support, int oversample,
+        int conv_size_half, const void* conv_func)
+{
+    if (num_w_planes < 1) return fail(MI355_ERR_INVALID_ARG, "num_w_planes must be >= 1");
+    if (num_w_planes > (1u << 24)) return fail(MI355_ERR_TOO_LARGE, "num_w_planes > 2^24");
+    if (!support) return fail(MI355_ERR_INVALID_ARG, "support is NULL");
+    if (!conv_func) return fail(MI355_ERR_INVALID_ARG, "conv_func is NULL");
+    if (oversample < 1) return fail(MI355_ERR_INVALID_ARG, "oversample must be >= 1");
+    if (conv_size_half < 1) return fail(MI355_ERR_INVALID_ARG, "conv_size_half must be >= 1");
+    const int half = (oversample + 1) / 2;
+    for (size_t p = 0; p < num_w_planes; ++p)
+    {
+        const int s = support[p];
+        if (s < 0 || s > kMaxSupport)
+            return fail(MI355_ERR_INVALID_ARG, "support[%zu] = %d outside [0, %d]", p, s, kMaxSupport);
+        // Largest kernel index read by the serial code: |off + ov*s| <= ov/2 + ov*s.
+        if ((long long)oversample * s + half >= conv_size_half)
+            return fail(MI355_ERR_KERNEL_TOO_SMALL,
+                    "plane %zu: support %d needs conv_size_half > %lld, have %d",
+                    p, s, (long long)oversample * s + half, conv_size_half);
+    }
+    return MI355_OK;
+}
+
+int check_grid_args(int grid_size)
+{
+    if (grid_size < 1) return fail(MI355_ERR_INVALID_ARG, "grid_size must be >= 1");
+    if (grid_size > (1 << 20)) return fail(MI355_ERR_TOO_LARGE, "grid_size > 2^20");
+    return MI355_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel tables
+// ---------------------------------------------------------------------------
+template <typename TI, typename TA>
+int prepare_tables(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
+        const typename V2<TI>::t* conv_dev)
+{
+    const int prec = (int)(16 * sizeof(TI) + sizeof(TA));
+    if (C->cache_kernels && C->tables_valid && C->cache_conv == conv_dev &&
+            C->cache_prec == prec && C->cache_ov == ov && C->cache_csh == csh &&
+            C->cache_support.size() == P &&
+            std::equal(support, support + P, C->cache_support.begin()))
+        return MI355_OK;
+    C->tables_valid = false;
+    const int half = (ov + 1) / 2, noff = half + 1;
+    C->plane_off.resize(P);
+    unsigned long long off = 0;
+    int smax = 0;
+    for (size_t p = 0; p < P; ++p)
+    {
+        C->plane_off[p] = (unsigned)off;
+        const unsigned long long W = 2ull * support[p] + 1;
+        off += W * W * noff * noff;
+        if (off > 0xFFFFFFFFull)
+            return fail(MI355_ERR_TOO_LARGE, "expanded kernel tables exceed 2^32 elements");
+        smax = std::max(smax, support[p]);
+    }
+    C->table_elems = off;
+    C->max_support = smax;
+    typedef typename V2<TA>::t T2;
+    // The tables are followed by a zero row read for rows outside a
+    // footprint; the whole buffer is addressed with 32-bit byte offsets.
+    const unsigned long long zero_elems = 2ull * kMaxSupport + 1;
+    if ((off + zero_elems) * sizeof(T2) > 0xFFFFFFFFull)
+        return fail(MI355_ERR_TOO_LARGE, "expanded kernel tables exceed 4 GiB");
+    TRY(ensure(C->tables, (off + zero_elems) * sizeof(T2)));
+    HIP_TRY(hipMemsetAsync(C->tables.as<T2>() + off, 0, zero_elems * sizeof(T2), C->stream));
+    TRY(ensure(C->norm_table, P * (2 * half + 1) * (2 * half + 1) * sizeof(double)));
+    TRY(ensure(C->support_d, P * sizeof(int)));
+    TRY(ensure(C->plane_off_d, P * sizeof(unsigned)));
+    C->cache_support.assign(support, support + P);
+    HIP_TRY(hipMemcpyAsync(C->support_d.p, C->cache_support.data(), P * sizeof(int),
+            hipMemcpyHostToDevice, C->stream));
+    HIP_TRY(hipMemcpyAsync(C->plane_off_d.p, C->plane_off.data(), P * sizeof(unsigned),
+            hipMemcpyHostToDevice, C->stream));
+    {
+        const unsigned long long Wm = 2ull * smax + 1;
+        const unsigned per_plane = (unsigned)(Wm * Wm * noff * noff);
+        dim3 g(grid_blocks(per_plane, 256, 1024), (unsigned)std::min<size_t>(P, 65535));
+        hipLaunchKernelGGL((k_expand_tables<TI, TA>), g, dim3(256), 0, C->stream,
+                conv_dev, (int)P, csh, ov, half, C->support_d.as<int>(),
+                C->plane_off_d.as<unsigned>(), C->tables.as<T2>());
+        HIP_TRY(hipGetLastError());
+        const unsigned entries = (unsigned)(P * (2 * half + 1) * (2 * half + 1));
+        hipLaunchKernelGGL(k_norm_table<TI>, dim3(grid_blocks(entries, 64, 65535)), dim3(64), 0,
+                C->stream, conv_dev, (int)P, csh, ov, half, C->support_d.as<int>(),
+                C->norm_table.as<double>());
+        HIP_TRY(hipGetLastError());
+    }
+    C->cache_conv = conv_dev;
+    C->cache_prec = prec;
+    C->cache_ov = ov;
+    C->cache_csh = csh;
+    C->tables_valid = true;
+    return MI355_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of n unsigned values (n given on the host or the device)
+// ---------------------------------------------------------------------------
+int scan_u32(mi355_ctx* C, const unsigned* in, unsigned* out, DevCount cnt,
+        unsigned long long n_bound, unsigned long long* d_total, unsigned* d_overflow,
+        int write_end)
+{
+    const unsigned long long nchunk = (n_bound + kScanChunk - 1) / kScanChunk;
+    TRY(ensure(C->scan_partial, (nchunk + 1) * sizeof(unsigned long long)));
+    unsigned long long* part = C->scan_partial.as<unsigned long long>();
+    const unsigned blocks = grid_blocks(nchunk, 1, 16384);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(blocks), dim3(kScanThreads), 0, C->stream, in, cnt, part);
+    hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(1024), 0, C->stream, cnt, part, d_total,
+            d_overflow, out, write_end);
+    hipLaunchKernelGGL(k_scan_down, dim3(blocks), dim3(kScanThreads), 0, C->stream, in, cnt, part, out);
+    HIP_TRY(hipGetLastError());
+    return MI355_OK;
+}
+
+// Device scalars (u64 slots in C->scalars).
+enum Scalar { SC_VISITS, SC_OVERFLOW, SC_ITEMS, SC_SLOTS, SC_SCRATCH0, SC_COUNT = 8 };
+
+// ---------------------------------------------------------------------------
+// One pass of the pipeline over num_jobs jobs.
+// ---------------------------------------------------------------------------
+// TI: precision of the caller's inputs and of selection; TA: accumulation.
+template <typename TI, typename TA>
+struct JobHost
+{
+    unsigned long long vis_offset, num_points;
+    TI cell_size_rad, w_scale;
+    int grid_size;
+    typename V2<TA>::t* grid;
+    double* norm;
+    unsigned long long* skipped;
+};
+
+// Called once the gridded bounding box of each job is known (before the
+// grid is touched); the host drop-in uploads the caller's grid window here.
+struct BeforeGrid
+{
+    virtual int operator()(mi355_ctx* C, const int4* bbox) = 0;
+    virtual ~BeforeGrid() {}
+};
+
+template <typename TI, typename TA>
+int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
+        const typename V2<TI>::t* conv, int num_jobs, const JobHost<TI, TA>* jobs,
+        const TI* uu, const TI* vv, const TI* ww, const TI* vis, const TI* weight,
+        BeforeGrid* before_grid)
+{
+    typedef TA T;
+    typedef typename V2<T>::t T2;
+    typedef typename V4<T>::t T4;
+    C->stats = mi355_stats();
+    record(C, EV_START);
+    TRY((prepare_tables<TI, TA>(C, P, support, ov, csh, conv)));
+    record(C, EV_PREP);
+    const int half = (ov + 1) / 2, noff = half + 1;
+
+    // Jobs.
+    C->h_jobs.resize(sizeof(JobDev<T>) * num_jobs);
+    JobDev<T>* hj = reinterpret_cast<JobDev<T>*>(C->h_jobs.data());
+    C->h_job_chunk_off.assign(num_jobs + 1, 0);
+    C->h_job_tile_base.assign(num_jobs + 1, 0);
+    C->h_job_vis_off.assign(num_jobs + 1, 0);
+    unsigned long long chunks = 0, tiles = 0, n_total = 0;
+    for (int j = 0; j < num_jobs; ++j)
+    {
+        const JobHost<TI, TA>& J = jobs[j];
+        TRY(check_grid_args(J.grid_size));
+        if (J.vis_offset != n_total)
+            return fail(MI355_ERR_INVALID_ARG, "job %d: visibilities must be contiguous and in job order", j);
+        JobDev<T> d;
+        d.grid = J.grid;
+        d.norm = J.norm;
+        d.skipped = J.skipped;
+        // grid_scale = grid_size * cell_size_rad in the input precision
+        // (oskar_grid_wproj.c:148 for float, :59 for double).
+        const TI gs = J.grid_size * J.cell_size_rad;
+        d.grid_scale_f = (float)gs;
+        d.w_scale_f = (float)J.w_scale;
+        d.grid_scale_d = (double)gs;
+        d.w_scale_d = (double)J.w_scale;
+        d.grid_size = J.grid_size;
+        d.ntx = (J.grid_size + kTileW - 1) / kTileW;
+        d.nty = (J.grid_size + kTileH - 1) / kTileH;
+        d.tile_base = (unsigned)tiles;
+        d.vis_begin = n_total;
+        d.vis_end = n_total + J.num_points;
+        d.chunk_base = (unsigned)chunks;
+        hj[j] = d;
+        C->h_job_chunk_off[j] = (unsigned)chunks;
+        C->h_job_tile_base[j] = (unsigned)tiles;
+        C->h_job_vis_off[j] = n_total;
+        chunks += (J.num_points + kSelChunk - 1) / kSelChunk;
+        tiles += (unsigned long long)d.ntx * d.nty;
+        n_total += J.num_points;
+        if (tiles > 0xFFFFFFFFull || chunks > 0xFFFFFFFFull)
+            return fail(MI355_ERR_TOO_LARGE, "too many tiles or visibilities in one call");
+    }
+    C->h_job_chunk_off[num_jobs] = (unsigned)chunks;
+    C->h_job_tile_base[num_jobs] = (unsigned)tiles;
+    C->h_job_vis_off[num_jobs] = n_total;
+    if (n_total >= 0xFFFFFFFFull)
+        return fail(MI355_ERR_TOO_LARGE, "more than 2^32-1 visibilities in one call");
+    const unsigned num_tiles = (unsigned)tiles;
+    const int tile_bits = std::max(1, bits_for(tiles));
+    if (tile_bits > 32) return fail(MI355_ERR_TOO_LARGE, "tile id exceeds 32 bits");
+    const unsigned long long num_tables = (unsigned long long)P * noff * noff;
+    const int table_bits_full = bits_for(num_tables);
+    const int table_id_shift = std::max(0, table_bits_full - 16);
+    const int key_table_bits = std::min(std::max(0, table_bits_full - table_id_shift), 32 - tile_bits);
+    const int key_table_shift = std::max(0, table_bits_full - table_id_shift - key_table_bits);
+
+    TRY(ensure(C->jobs_d, C->h_jobs.size()));
+    TRY(ensure(C->job_chunk_off_d, (num_jobs + 1) * sizeof(unsigned)));
+    TRY(ensure(C->job_tile_base_d, (num_jobs + 1) * sizeof(unsigned)));
+    TRY(ensure(C->job_vis_off_d, (num_jobs + 1) * sizeof(unsigned long long)));
+    TRY(ensure(C->job_out_d, 2 * num_jobs * sizeof(unsigned long long)));
+    TRY(ensure(C->job_bbox_d, num_jobs * sizeof(int4)));
+    HIP_TRY(hipMemcpyAsync(C->jobs_d.p, C->h_jobs.data(), C->h_jobs.size(), hipMemcpyHostToDevice, C->stream));
+    HIP_TRY(hipMemcpyAsync(C->job_chunk_off_d.p, C->h_job_chunk_off.data(), (num_jobs + 1) * sizeof(unsigned), hipMemcpyHostToDevice, C->stream));
+    HIP_TRY(hipMemcpyAsync(C->job_tile_base_d.p, C->h_job_tile_base.data(), (num_jobs + 1) * sizeof(unsigned), hipMemcpyHostToDevice, C->stream));
+    HIP_TRY(hipMemcpyAsync(C->job_vis_off_d.p, C->h_job_vis_off.data(), (num_jobs + 1) * sizeof(unsigned long long), hipMemcpyHostToDevice, C->stream));
+
+    // Selection.
+    TRY(ensure(C->meta, n_total * sizeof(int4)));
+    TRY(ensure(C->amp, n_total * sizeof(T4)));
+    TRY(ensure(C->num_visits, n_total * sizeof(unsigned)));
+    TRY(ensure(C->visit_off, (n_total + 1) * sizeof(unsigned)));
+    TRY(ensure(C->part_norm, chunks * sizeof(double)));
+    TRY(ensure(C->part_skipped, chunks * sizeof(unsigned long long)));
+    TRY(ensure(C->part_taps, chunks * sizeof(unsigned long long)));
+    TRY(ensure(C->part_bbox, chunks * sizeof(int4)));
+    TRY(ensure(C->scalars, SC_COUNT * sizeof(unsigned long long)));
+    unsigned long long* sc = C->scalars.as<unsigned long long>();
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, C->stream,
+            reinterpret_cast<unsigned*>(sc), (unsigned long long)(2 * SC_COUNT), 0u);
+
+    SelectArgs<TI, TA> sa;
+    sa.uu = uu; sa.vv = vv; sa.ww = ww; sa.vis = vis; sa.weight = weight;
+    sa.jobs = C->jobs_d.as<JobDev<T>>(); sa.num_jobs = num_jobs;
+    sa.job_chunk_off = C->job_chunk_off_d.as<unsigned>();
+    sa.num_planes = (int)P; sa.support = C->support_d.as<int>();
+    sa.oversample = ov; sa.half_ov = half;
+    sa.norm_table = C->norm_table.as<double>();
+    sa.plane_off = C->plane_off_d.as<unsigned>();
+    sa.table_id_shift = table_id_shift;
+    sa.meta = C->meta.as<int4>(); sa.amp = C->amp.as<T4>();
+    sa.num_visits = C->num_visits.as<unsigned>();
+    sa.part_norm = C->part_norm.as<double>();
+    sa.part_skipped = C->part_skipped.as<unsigned long long>();
+    sa.part_taps = C->part_taps.as<unsigned long long>();
+    sa.part_bbox = C->part_bbox.as<int4>();
+    if (chunks)
+    {
+        hipLaunchKernelGGL((k_select<TI, TA>), dim3(grid_blocks(chunks, 1, 65535)), dim3(kSelThreads), 0,
+                C->stream, sa, (unsigned)chunks);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_job_reduce<T>, dim3(num_jobs), dim3(kSelThreads), 0, C->stream,
+            C->jobs_d.as<JobDev<T>>(), C->job_chunk_off_d.as<unsigned>(),
+            C->part_norm.as<double>(), C->part_skipped.as<unsigned long long>(),
+            C->part_taps.as<unsigned long long>(), C->part_bbox.as<int4>(),
+            C->job_out_d.as<unsigned long long>(), C->job_bbox_d.as<int4>());
+    HIP_TRY(hipGetLastError());
+    record(C, EV_SELECT);
+
+    // Visit offsets and total.
+    {
+        DevCount cnt{nullptr, 1, 0, n_total};
+        TRY(scan_u32(C, C->num_visits.as<unsigned>(), C->visit_off.as<unsigned>(), cnt,
+                n_total, sc + SC_VISITS, reinterpret_cast<unsigned*>(sc + SC_OVERFLOW), 1));
+    }
+    unsigned long long host_sc[SC_COUNT];
+    C->h_job_out.resize(2 * num_jobs);
+    C->h_job_bbox.resize(num_jobs);
+    HIP_TRY(hipMemcpyAsync(host_sc, sc, sizeof(host_sc), hipMemcpyDeviceToHost, C->stream));
+    HIP_TRY(hipMemcpyAsync(C->h_job_out.data(), C->job_out_d.p, 2 * num_jobs * sizeof(unsigned long long), hipMemcpyDeviceToHost, C->stream));
+    HIP_TRY(hipMemcpyAsync(C->h_job_bbox.data(), C->job_bbox_d.p, num_jobs * sizeof(int4), hipMemcpyDeviceToHost, C->stream));
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    if (host_sc[SC_OVERFLOW]) return fail(MI355_ERR_TOO_LARGE, "more than 2^32-1 tile visits in one call");
+    const unsigned long long T_vis = host_sc[SC_VISITS];
+    C->stats.num_points = n_total;
+    for (int j = 0; j < num_jobs; ++j)
+    {
+        C->stats.num_skipped += C->h_job_out[2 * j];
+        C->stats.num_taps += C->h_job_out[2 * j + 1];
+    }
+    C->stats.num_visits = T_vis;
+    if (before_grid) TRY((*before_grid)(C, C->h_job_bbox.data()));
+    if (T_vis == 0)
+    {
+        record(C, EV_BUCKET); record(C, EV_GRID); record(C, EV_MERGE);
+        return MI355_OK;
+    }
+
+    // Emit (tile | table) keys and sort them.
+    for (int b = 0; b < 2; ++b)
+    {
+        TRY(ensure(C->keys[b], T_vis * sizeof(unsigned)));
+        TRY(ensure(C->vals[b], T_vis * sizeof(unsigned)));
+    }
+    hipLaunchKernelGGL(k_emit<T>, dim3(grid_blocks(n_total, 256, 16384)), dim3(256), 0, C->stream,
+            C->meta.as<int4>(), C->num_visits.as<unsigned>(), C->visit_off.as<unsigned>(),
+            C->jobs_d.as<JobDev<T>>(), C->job_vis_off_d.as<unsigned long long>(), num_jobs,
+            n_total, key_table_shift, key_table_bits, C->keys[0].as<unsigned>(), C->vals[0].as<unsigned>());
+    HIP_TRY(hipGetLastError());
+    const int key_bits = tile_bits + key_table_bits;
+    const unsigned long long nchunk = (T_vis + kSortChunk - 1) / kSortChunk;
+    TRY(ensure(C->hist, 256 * nchunk * sizeof(unsigned)));
+    TRY(ensure(C->hist_off, (256 * nchunk + 1) * sizeof(unsigned)));
+    int cur = 0;
+    for (int shift = 0; shift < key_bits; shift += 8)
+    {
+        const unsigned blocks = grid_blocks(nchunk, 1, 16384);
+        hipLaunchKernelGGL(k_sort_hist, dim3(blocks), dim3(kSortThreads), 0, C->stream,
+                C->keys[cur].as<unsigned>(), sc + SC_VISITS, shift, C->hist.as<unsigned>());
+        DevCount hc{sc + SC_VISITS, (unsigned long long)kSortChunk, 256, 0};
+        TRY(scan_u32(C, C->hist.as<unsigned>(), C->hist_off.as<unsigned>(), hc, 256 * nchunk,
+                sc + SC_SCRATCH0, reinterpret_cast<unsigned*>(sc + SC_SCRATCH0 + 1), 0));
+        hipLaunchKernelGGL(k_sort_scatter, dim3(blocks), dim3(kSortThreads), 0, C->stream,
+                C->keys[cur].as<unsigned>(), C->vals[cur].as<unsigned>(), sc + SC_VISITS, shift,
+                C->hist_off.as<unsigned>(), C->keys[cur ^ 1].as<unsigned>(), C->vals[cur ^ 1].as<unsigned>());
+        HIP_TRY(hipGetLastError());
+        cur ^= 1;
+    }
+    const unsigned* skeys = C->keys[cur].as<unsigned>();
+    const unsigned* svals = C->vals[cur].as<unsigned>();
+
+    // Tiles -> work items.
+    TRY(ensure(C->tile_start, num_tiles * sizeof(unsigned)));
+    TRY(ensure(C->tile_end, num_tiles * sizeof(unsigned)));
+    TRY(ensure(C->n_items, num_tiles * sizeof(unsigned)));
+    TRY(ensure(C->n_slots, num_tiles * sizeof(unsigned)));
+    TRY(ensure(C->item_off, (num_tiles + 1) * sizeof(unsigned)));
+    TRY(ensure(C->slot_off, (num_tiles + 1) * sizeof(unsigned)));
+    const unsigned tb = grid_blocks(num_tiles, 256, 16384);
+    hipLaunchKernelGGL(k_fill_u32, dim3(tb), dim3(256), 0, C->stream, C->tile_start.as<unsigned>(), (unsigned long long)num_tiles, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(tb), dim3(256), 0, C->stream, C->tile_end.as<unsigned>(), (unsigned long long)num_tiles, 0u);
+    hipLaunchKernelGGL(k_tile_bounds, dim3(grid_blocks(T_vis, 256, 65535)), dim3(256), 0, C->stream,
+            skeys, sc + SC_VISITS, key_table_bits, C->tile_start.as<unsigned>(), C->tile_end.as<unsigned>());
+    hipLaunchKernelGGL(k_items, dim3(tb), dim3(256), 0, C->stream, C->tile_start.as<unsigned>(),
+            C->tile_end.as<unsigned>(), num_tiles, C->visits_per_item, C->n_items.as<unsigned>(),
+            C->n_slots.as<unsigned>());
+    HIP_TRY(hipGetLastError());
+    {
+        DevCount cnt{nullptr, 1, 0, num_tiles};
+        TRY(scan_u32(C, C->n_items.as<unsigned>(), C->item_off.as<unsigned>(), cnt, num_tiles,
+                sc + SC_ITEMS, reinterpret_cast<unsigned*>(sc + SC_OVERFLOW), 1));
+        TRY(scan_u32(C, C->n_slots.as<unsigned>(), C->slot_off.as<unsigned>(), cnt, num_tiles,
+                sc + SC_SLOTS, reinterpret_cast<unsigned*>(sc + SC_OVERFLOW), 1));
+    }
+    HIP_TRY(hipMemcpyAsync(host_sc, sc, sizeof(host_sc), hipMemcpyDeviceToHost, C->stream));
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    const unsigned long long num_items = host_sc[SC_ITEMS], num_slots = host_sc[SC_SLOTS];
+    TRY(ensure(C->item_tile, num_items * sizeof(unsigned)));
+    hipLaunchKernelGGL(k_item_map, dim3(tb), dim3(256), 0, C->stream, C->item_off.as<unsigned>(),
+            num_tiles, C->item_tile.as<unsigned>());
+    HIP_TRY(hipGetLastError());
+    record(C, EV_BUCKET);
+    C->stats.num_items = num_items;
+
+    // Grid.
+    const bool comp = C->compensated;
+    const size_t part_bytes = comp ? sizeof(double2) : sizeof(T2);
+    TRY(ensure(C->scratch, num_slots * kTileCells * part_bytes));
+    GridArgs<T> ga;
+    ga.tables = C->tables.as<T2>();
+    ga.table_bytes = (unsigned)((C->table_elems + 2ull * kMaxSupport + 1) * sizeof(T2));
+    ga.zero_row_byte = (unsigned)(C->table_elems * sizeof(T2));
+    ga.meta = C->meta.as<int4>();
+    ga.amp = C->amp.as<T4>();
+    ga.visit_vis = svals;
+    ga.tile_start = C->tile_start.as<unsigned>();
+    ga.tile_end = C->tile_end.as<unsigned>();
+    ga.item_off = C->item_off.as<unsigned>();
+    ga.slot_off = C->slot_off.as<unsigned>();
+    ga.item_tile = C->item_tile.as<unsigned>();
+    ga.d_num_items = reinterpret_cast<const unsigned*>(C->item_off.as<unsigned>() + num_tiles);
+    ga.jobs = C->jobs_d.as<JobDev<T>>();
+    ga.job_tile_base = C->job_tile_base_d.as<unsigned>();
+    ga.num_jobs = num_jobs;
+    ga.visits_per_item = C->visits_per_item;
+    ga.scratch = C->scratch.p;
+    const unsigned padded = (unsigned)((num_items + 7) & ~7ull);
+    if (num_items)
+    {
+        if (comp)
+        {
+            const size_t lds = (size_t)kWaves * kRowsPerWave * 64 * 4 * sizeof(T);
+            HIP_TRY(hipFuncSetAttribute((const void*)k_grid_tiles<T, true>,
+                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL((k_grid_tiles<T, true>), dim3(padded), dim3(kBlock), lds, C->stream, ga, padded);
+        }
+        else
+            hipLaunchKernelGGL((k_grid_tiles<T, false>), dim3(padded), dim3(kBlock), 0, C->stream, ga, padded);
+        HIP_TRY(hipGetLastError());
+    }
+    record(C, EV_GRID);
+    if (num_slots)
+    {
+        if (comp)
+            hipLaunchKernelGGL((k_merge<T, double2>), dim3(std::min(num_tiles, 65535u)), dim3(256), 0, C->stream,
+                    C->item_off.as<unsigned>(), C->slot_off.as<unsigned>(), num_tiles,
+                    C->jobs_d.as<JobDev<T>>(), C->job_tile_base_d.as<unsigned>(), num_jobs,
+                    C->scratch.as<double2>());
+        else
+            hipLaunchKernelGGL((k_merge<T, T2>), dim3(std::min(num_tiles, 65535u)), dim3(256), 0, C->stream,
+                    C->item_off.as<unsigned>(), C->slot_off.as<unsigned>(), num_tiles,
+                    C->jobs_d.as<JobDev<T>>(), C->job_tile_base_d.as<unsigned>(), num_jobs,
+                    C->scratch.as<T2>());
+        HIP_TRY(hipGetLastError());
+    }
+    record(C, EV_MERGE);
+    return MI355_OK;
+}
+
+void collect_timing(mi355_ctx* C)
+{
+    if (!C->timing) return;
+    if (hipEventSynchronize(C->ev[EV_MERGE]) != hipSuccess) { (void)hipGetLastError(); return; }
+    float ms[EV_COUNT] = {};
+    for (int i = 1; i < EV_COUNT; ++i)
+        if (hipEventElapsedTime(&ms[i], C->ev[i - 1], C->ev[i]) != hipSuccess)
+        { (void)hipGetLastError(); ms[i] = 0; }
+    C->stats.ms_prepare = ms[EV_PREP];
+    C->stats.ms_select = ms[EV_SELECT];
+    C->stats.ms_bucket = ms[EV_BUCKET];
+    C->stats.ms_grid = ms[EV_GRID];
+    C->stats.ms_merge = ms[EV_MERGE];
+    float tot = 0;
+    if (hipEventElapsedTime(&tot, C->ev[EV_START], C->ev[EV_MERGE]) != hipSuccess)
+    { (void)hipGetLastError(); tot = 0; }
+    C->stats.ms_total = tot;
+}
+
+// ---------------------------------------------------------------------------
+// Device-pointer gridding, one job.
+// ---------------------------------------------------------------------------
+template <typename TI, typename TA>
+int grid_dev(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
+        const TI* conv, size_t N, const TI* uu, const TI* vv, const TI* ww,
+        const TI* vis, const TI* weight, TI cell, TI w_scale, int G,
+        unsigned long long* skipped, double* norm, TA* grid, BeforeGrid* hook)
+{
+    TRY(check_kernel_args(P, support, ov, csh, conv));
+    TRY(check_grid_args(G));
+    if (!skipped || !norm || !grid) return fail(MI355_ERR_INVALID_ARG, "NULL output pointer");
+    if (N && (!uu || !vv || !ww || !vis || !weight))
+        return fail(MI355_ERR_INVALID_ARG, "NULL visibility array");
+    HIP_TRY(hipSetDevice(C->device));
+    JobHost<TI, TA> J;
+    J.vis_offset = 0; J.num_points = N;
+    J.cell_size_rad = cell; J.w_scale = w_scale; J.grid_size = G;
+    J.grid = reinterpret_cast<typename V2<TA>::t*>(grid);
+    J.norm = norm; J.skipped = skipped;
+    TRY((run_pipeline<TI, TA>(C, P, support, ov, csh,
+            reinterpret_cast<const typename V2<TI>::t*>(conv), 1, &J, uu, vv, ww, vis, weight,
+            hook)));
+    collect_timing(C);
+    return ok();
+}
+
+// Per-visibility selection only (one job): out is int32 [8][N].
+template <typename TI>
+int select_dev(mi355_ctx* C, size_t P, const int* support, int ov, size_t N,
+        const TI* uu, const TI* vv, const TI* ww, TI cell, TI w_scale, int G, int* out)
+{
+    if (P < 1 || !support || ov < 1) return fail(MI355_ERR_INVALID_ARG, "bad kernel arguments");
+    TRY(check_grid_args(G));
+    if (N && (!uu || !vv || !ww || !out)) return fail(MI355_ERR_INVALID_ARG, "NULL pointer argument");
+    if (N == 0) return ok();
+    HIP_TRY(hipSetDevice(C->device));
+    TRY(ensure(C->support_d, P * sizeof(int)));
+    C->cache_support.assign(support, support + P);
+    C->tables_valid = false;
+    HIP_TRY(hipMemcpyAsync(C->support_d.p, C->cache_support.data(), P * sizeof(int),
+            hipMemcpyHostToDevice, C->stream));
+    const TI gs = G * cell;
+    hipLaunchKernelGGL(k_select_out<TI>, dim3(grid_blocks(N, 256, 65535)), dim3(256), 0,
+            C->stream, uu, vv, ww, (unsigned long long)N, gs, w_scale, G, ov, (int)P,
+            C->support_d.as<int>(), out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    return ok();
+}
+
+// ---------------------------------------------------------------------------
+// Host-pointer drop-in.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct UploadWindow : BeforeGrid
+{
+    const T* host_grid; int G; int4 box; bool any = false;
+    int operator()(mi355_ctx* C, const int4* bbox) override
+    {
+        box = bbox[0];
+        any = box.x <= box.z && box.y <= box.w;
+        if (!any) return MI355_OK;
+        typedef typename V2<T>::t T2;
+        const size_t pitch = (size_t)G * sizeof(T2);
+        const size_t off = (size_t)box.y * G + box.x;
+        HIP_TRY(hipMemcpy2DAsync(C->in_grid.as<T2>() + off, pitch,
+                reinterpret_cast<const T2*>(host_grid) + off, pitch,
+                (size_t)(box.z - box.x + 1) * sizeof(T2), (size_t)(box.w - box.y + 1),
+                hipMemcpyHostToDevice, C->stream));
+        return MI355_OK;
+    }
+};
+
+template <typename T>
+int grid_host(size_t P, const int* support, int ov, int csh, const T* conv,
+        size_t N, const T* uu, const T* vv, const T* ww, const T* vis,
+        const T* weight, T cell, T w_scale, int G, size_t* num_skipped,
+        double* norm, T* grid)
+{
+    TRY(check_kernel_args(P, support, ov, csh, conv));
+    TRY(check_grid_args(G));
+    if (!num_skipped || !norm || !grid) return fail(MI355_ERR_INVALID_ARG, "NULL output pointer");
+    if (N && (!uu || !vv || !ww || !vis || !weight))
+        return fail(MI355_ERR_INVALID_ARG, "NULL visibility array");
+    if (N == 0) { *num_skipped = 0; return ok(); }   // oskar_grid_wproj.c:151
+    std::lock_guard<std::mutex> lock(g_default_lock);
+    mi355_ctx* C = nullptr;
+    TRY(get_default_ctx(&C));
+    HIP_TRY(hipSetDevice(C->device));
+    typedef typename V2<T>::t T2;
+    const size_t conv_n = (size_t)P * csh * csh * 2;
+    TRY(ensure(C->in_conv, conv_n * sizeof(T)));
+    TRY(ensure(C->in_uu, N * sizeof(T)));
+    TRY(ensure(C->in_vv, N * sizeof(T)));
+    TRY(ensure(C->in_ww, N * sizeof(T)));
+    TRY(ensure(C->in_vis, 2 * N * sizeof(T)));
+    TRY(ensure(C->in_weight, N * sizeof(T)));
+    TRY(ensure(C->in_grid, (size_t)G * G * sizeof(T2)));
+    TRY(ensure(C->in_norm, sizeof(double)));
+    TRY(ensure(C->in_skipped, sizeof(unsigned long long)));
+    HIP_TRY(hipMemcpyAsync(C->in_conv.p, conv, conv_n * sizeof(T), hipMemcpyHostToDevice, C->stream));
+    HIP_TRY(hipMemcpyAsync(C->in_uu.p, uu, N * sizeof(T), hipMemcpyHostToDevice, C->stream));
+    HIP_TRY(hipMemcpyAsync(C->in_vv.p, vv, N * sizeof(T), hipMemcpyHostToDevice, C->stream));
+    HIP_TRY(hipMemcpyAsync(C->in_ww.p, ww, N * sizeof(T), hipMemcpyHostToDevice, C->stream));
+    HIP_TRY(hipMemcpyAsync(C->in_vis.p, vis, 2 * N * sizeof(T), hipMemcpyHostToDevice, C->stream));
+    HIP_TRY(hipMemcpyAsync(C->in_weight.p, weight, N * sizeof(T), hipMemcpyHostToDevice, C->stream));
+    const double zero = 0.0;
+    HIP_TRY(hipMemcpyAsync(C->in_norm.p, &zero, sizeof(double), hipMemcpyHostToDevice, C->stream));
+    // The host conv_func may change between calls at the same address.
+    C->tables_valid = false;
+    UploadWindow<T> up;
+    up.host_grid = grid;
+    up.G = G;
+    TRY((grid_dev<T, T>(C, P, support, ov, csh, C->in_conv.as<T>(), N, C->in_uu.as<T>(),
+            C->in_vv.as<T>(), C->in_ww.as<T>(), C->in_vis.as<T>(), C->in_weight.as<T>(),
+            cell, w_scale, G, C->in_skipped.as<unsigned long long>(), C->in_norm.as<double>(),
+            C->in_grid.as<T>(), &up)));
+    if (up.any)
+    {
+        const size_t pitch = (size_t)G * sizeof(T2);
+        const size_t off = (size_t)up.box.y * G + up.box.x;
+        HIP_TRY(hipMemcpy2DAsync(reinterpret_cast<T2*>(grid) + off, pitch,
+                C->in_grid.as<T2>() + off, pitch, (size_t)(up.box.z - up.box.x + 1) * sizeof(T2),
+                (size_t)(up.box.w - up.box.y + 1), hipMemcpyDeviceToHost, C->stream));
+    }
+    double dnorm = 0.0;
+    unsigned long long dskip = 0;
+    HIP_TRY(hipMemcpyAsync(&dnorm, C->in_norm.p, sizeof(double), hipMemcpyDeviceToHost, C->stream));
+    HIP_TRY(hipMemcpyAsync(&dskip, C->in_skipped.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, C->stream));
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    *norm += dnorm;
+    *num_skipped = (size_t)dskip;
+    return ok();
+}
+
+// ---------------------------------------------------------------------------
+// Degridding (adjoint), device pointers.
+// ---------------------------------------------------------------------------
+template <typename T>
+int degrid_dev(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
+        const T* conv, size_t N, const T* uu, const T* vv, const T* ww,
+        const T* weight, T cell, T w_scale, int G, const T* grid,
+        int apply_weight, T* out)
+{
+    TRY(check_kernel_args(P, support, ov, csh, conv));
+    TRY(check_grid_args(G));
+    if (!grid || (N && (!out || !uu || !vv || !ww || !weight)))
+        return fail(MI355_ERR_INVALID_ARG, "NULL pointer argument");
+    if (N >= 0xFFFFFFFFull) return fail(MI355_ERR_TOO_LARGE, "too many visibilities");
+    if (N == 0) return ok();
+    HIP_TRY(hipSetDevice(C->device));
+    typedef typename V2<T>::t T2;
+    typedef typename V4<T>::t T4;
+    TRY((prepare_tables<T, T>(C, P, support, ov, csh, reinterpret_cast<const T2*>(conv))));
+    const int half = (ov + 1) / 2, noff = half + 1;
+    // One job; its norm/skipped totals go to library scratch (not reported).
+    TRY(ensure(C->in_norm, sizeof(double)));
+    TRY(ensure(C->in_skipped, sizeof(unsigned long long)));
+    C->h_jobs.resize(sizeof(JobDev<T>));
+    JobDev<T>* hj = reinterpret_cast<JobDev<T>*>(C->h_jobs.data());
+    hj->grid = const_cast<T2*>(reinterpret_cast<const T2*>(grid));
+    hj->norm = C->in_norm.as<double>();
+    hj->skipped = C->in_skipped.as<unsigned long long>();
+    const T gs = G * cell;
+    hj->grid_scale_f = (float)gs;
+    hj->w_scale_f = (float)w_scale;
+    hj->grid_scale_d = (double)gs;
+    hj->w_scale_d = (double)w_scale;
+    hj->grid_size = G;
+    hj->ntx = (G + kTileW - 1) / kTileW;
+    hj->nty = (G + kTileH - 1) / kTileH;
+    hj->tile_base = 0;
+    hj->vis_begin = 0;
+    hj->vis_end = N;
+    hj->chunk_base = 0;
+    const unsigned long long chunks = (N + kSelChunk - 1) / kSelChunk;
+    C->h_job_chunk_off.assign(2, 0);
+    C->h_job_chunk_off[1] = (unsigned)chunks;
+    TRY(ensure(C->jobs_d, sizeof(JobDev<T>)));
+    TRY(ensure(C->job_chunk_off_d, 2 * sizeof(unsigned)));
+    HIP_TRY(hipMemcpyAsync(C->jobs_d.p, hj, sizeof(JobDev<T>), hipMemcpyHostToDevice, C->stream));
+    HIP_TRY(hipMemcpyAsync(C->job_chunk_off_d.p, C->h_job_chunk_off.data(), 2 * sizeof(unsigned), hipMemcpyHostToDevice, C->stream));
+    TRY(ensure(C->meta, N * sizeof(int4)));
+    TRY(ensure(C->amp, N * sizeof(T4)));
+    TRY(ensure(C->num_visits, N * sizeof(unsigned)));
+    TRY(ensure(C->part_norm, chunks * sizeof(double)));
+    TRY(ensure(C->part_skipped, chunks * sizeof(unsigned long long)));
+    TRY(ensure(C->part_taps, chunks * sizeof(unsigned long long)));
+    TRY(ensure(C->part_bbox, chunks * sizeof(int4)));
+    SelectArgs<T, T> sa;
+    sa.uu = uu; sa.vv = vv; sa.ww = ww; sa.weight = weight;
+    sa.vis = nullptr;      // the adjoint needs the selection only, no amplitudes
+    sa.jobs = C->jobs_d.as<JobDev<T>>(); sa.num_jobs = 1;
+    sa.job_chunk_off = C->job_chunk_off_d.as<unsigned>();
+    sa.num_planes = (int)P; sa.support = C->support_d.as<int>();
+    sa.oversample = ov; sa.half_ov = half;
+    sa.norm_table = C->norm_table.as<double>();
+    sa.plane_off = C->plane_off_d.as<unsigned>();
+    sa.table_id_shift = std::max(0, bits_for((unsigned long long)P * noff * noff) - 16);
+    sa.meta = C->meta.as<int4>(); sa.amp = C->amp.as<T4>();
+    sa.num_visits = C->num_visits.as<unsigned>();
+    sa.part_norm = C->part_norm.as<double>();
+    sa.part_skipped = C->part_skipped.as<unsigned long long>();
+    sa.part_taps = C->part_taps.as<unsigned long long>();
+    sa.part_bbox = C->part_bbox.as<int4>();
+    hipLaunchKernelGGL((k_select<T, T>), dim3(grid_blocks(chunks, 1, 65535)), dim3(kSelThreads), 0,
+            C->stream, sa, (unsigned)chunks);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_degrid<T>, dim3(grid_blocks(N, 4, 65535)), dim3(256), 0, C->stream,
+            C->tables.as<T2>(), C->meta.as<int4>(), weight, (unsigned long long)N,
+            reinterpret_cast<const T2*>(grid), G, apply_weight, reinterpret_cast<T2*>(out));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    return ok();
+}
+
+} // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+MI355_EXPORT int mi355_grid_last_error(void) { return t_last_status; }
+MI355_EXPORT const char* mi355_grid_last_error_message(void) { return t_last_msg; }
+
+MI355_EXPORT const char* mi355_grid_error_string(int status)
+{
+    switch (status)
+    {
+    case MI355_OK: return "success";
+    case MI355_ERR_INVALID_ARG: return "invalid argument";
+    case MI355_ERR_KERNEL_TOO_SMALL: return "conv_size_half too small for the support table";
+    case MI355_ERR_NO_DEVICE: return "no HIP device";
+    case MI355_ERR_HIP: return "HIP runtime error";
+    case MI355_ERR_OUT_OF_MEMORY: return "device out of memory";
+    case MI355_ERR_TOO_LARGE: return "problem exceeds 32-bit indexing";
+    default: return "unknown status";
+    }
+}
+
+MI355_EXPORT void mi355_grid_wproj_d(const size_t num_w_planes, const int* support,
+        const int oversample, const int conv_size_half, const double* conv_func,
+        const size_t num_points, const double* uu, const double* vv,
+        const double* ww, const double* vis, const double* weight,
+        const double cell_size_rad, const double w_scale, const int grid_size,
+        size_t* num_skipped, double* norm, double* grid)
+{
+    (void)grid_host<double>(num_w_planes, support, oversample, conv_size_half, conv_func,
+            num_points, uu, vv, ww, vis, weight, cell_size_rad, w_scale, grid_size,
+            num_skipped, norm, grid);
+}
+
+MI355_EXPORT void mi355_grid_wproj_f(const size_t num_w_planes, const int* support,
+        const int oversample, const int conv_size_half, const float* conv_func,
+        const size_t num_points, const float* uu, const float* vv,
+        const float* ww, const float* vis, const float* weight,
+        const float cell_size_rad, const float w_scale, const int grid_size,
+        size_t* num_skipped, double* norm, float* grid)
+{
+    (void)grid_host<float>(num_w_planes, support, oversample, conv_size_half, conv_func,
+            num_points, uu, vv, ww, vis, weight, cell_size_rad, w_scale, grid_size,
+            num_skipped, norm, grid);
+}
+
+MI355_EXPORT int mi355_ctx_create(int device, mi355_ctx** ctx)
+{
+    if (!ctx) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    *ctx = nullptr;
+    mi355_ctx* C = new mi355_ctx();
+    const int s = init_ctx(C, device);
+    if (s != MI355_OK) { delete C; return s; }
+    *ctx = C;
+    return ok();
+}
+
+MI355_EXPORT int mi355_ctx_destroy(mi355_ctx* C)
+{
+    if (!C) return ok();
+    (void)hipSetDevice(C->device);
+    (void)hipStreamSynchronize(C->stream);
+    DevBuf* all[] = {&C->tables, &C->norm_table, &C->support_d, &C->plane_off_d, &C->meta,
+        &C->amp, &C->num_visits, &C->visit_off, &C->part_norm, &C->part_skipped,
+        &C->part_taps, &C->part_bbox, &C->scan_partial, &C->scalars, &C->keys[0], &C->keys[1],
+        &C->vals[0], &C->vals[1], &C->hist, &C->hist_off, &C->tile_start, &C->tile_end,
+        &C->n_items, &C->n_slots, &C->item_off, &C->slot_off, &C->item_tile, &C->scratch,
+        &C->jobs_d, &C->job_chunk_off_d, &C->job_vis_off_d, &C->job_tile_base_d,
+        &C->job_out_d, &C->job_bbox_d, &C->in_conv, &C->in_uu, &C->in_vv, &C->in_ww,
+        &C->in_vis, &C->in_weight, &C->in_grid, &C->in_norm, &C->in_skipped};
+    for (DevBuf* b : all) release(*b);
+    for (int i = 0; i < EV_COUNT; ++i) if (C->ev[i]) (void)hipEventDestroy(C->ev[i]);
+    delete C;
+    return ok();
+}
+
+MI355_EXPORT int mi355_ctx_set_stream(mi355_ctx* C, void* stream)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    C->stream = reinterpret_cast<hipStream_t>(stream);
+    return ok();
+}
+
+MI355_EXPORT int mi355_ctx_set_option(mi355_ctx* C, int key, long long value)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    switch (key)
+    {
+    case MI355_OPT_VISITS_PER_ITEM:
+        if (value < 1 || value > (1 << 24)) return fail(MI355_ERR_INVALID_ARG, "visits per item %lld", value);
+        C->visits_per_item = (unsigned)value;
+        break;
+    case MI355_OPT_COMPENSATED: C->compensated = value != 0; break;
+    case MI355_OPT_PHASE_TIMING: C->timing = value != 0; break;
+    case MI355_OPT_CACHE_KERNELS: C->cache_kernels = value != 0; C->tables_valid = false; break;
+    default: return fail(MI355_ERR_INVALID_ARG, "unknown option %d", key);
+    }
+    return ok();
+}
+
+MI355_EXPORT int mi355_ctx_get_stats(const mi355_ctx* C, mi355_stats* stats)
+{
+    if (!C || !stats) return fail(MI355_ERR_INVALID_ARG, "NULL argument");
+    *stats = C->stats;
+    return ok();
+}
+
+MI355_EXPORT int mi355_default_ctx(mi355_ctx** ctx)
+{
+    if (!ctx) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lock(g_default_lock);
+    TRY(get_default_ctx(ctx));
+    return ok();
+}
+
+MI355_EXPORT int mi355_grid_wproj_f_dev(mi355_ctx* C, size_t P, const int* support,
+        int ov, int csh, const float* conv, size_t N, const float* uu, const float* vv,
+        const float* ww, const float* vis, const float* weight, float cell, float w_scale,
+        int G, unsigned long long* skipped, double* norm, float* grid)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    return grid_dev<float, float>(C, P, support, ov, csh, conv, N, uu, vv, ww, vis, weight, cell,
+            w_scale, G, skipped, norm, grid, nullptr);
+}
+
+MI355_EXPORT int mi355_grid_wproj_fsel_d_dev(mi355_ctx* C, size_t P, const int* support,
+        int ov, int csh, const float* conv, size_t N, const float* uu, const float* vv,
+        const float* ww, const float* vis, const float* weight, float cell, float w_scale,
+        int G, unsigned long long* skipped, double* norm, double* grid)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    return grid_dev<float, double>(C, P, support, ov, csh, conv, N, uu, vv, ww, vis, weight,
+            cell, w_scale, G, skipped, norm, grid, nullptr);
+}
+
+MI355_EXPORT int mi355_select_wproj_f_dev(mi355_ctx* C, size_t P, const int* support,
+        int ov, size_t N, const float* uu, const float* vv, const float* ww, float cell,
+        float w_scale, int G, int* out)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    return select_dev<float>(C, P, support, ov, N, uu, vv, ww, cell, w_scale, G, out);
+}
+
+MI355_EXPORT int mi355_select_wproj_d_dev(mi355_ctx* C, size_t P, const int* support,
+        int ov, size_t N, const double* uu, const double* vv, const double* ww, double cell,
+        double w_scale, int G, int* out)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    return select_dev<double>(C, P, support, ov, N, uu, vv, ww, cell, w_scale, G, out);
+}
+
+MI355_EXPORT int mi355_grid_wproj_d_dev(mi355_ctx* C, size_t P, const int* support,
+        int ov, int csh, const double* conv, size_t N, const double* uu, const double* vv,
+        const double* ww, const double* vis, const double* weight, double cell,
+        double w_scale, int G, unsigned long long* skipped, double* norm, double* grid)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    return grid_dev<double, double>(C, P, support, ov, csh, conv, N, uu, vv, ww, vis, weight,
+            cell, w_scale, G, skipped, norm, grid, nullptr);
+}
+
+MI355_EXPORT int mi355_degrid_wproj_f_dev(mi355_ctx* C, size_t P, const int* support,
+        int ov, int csh, const float* conv, size_t N, const float* uu, const float* vv,
+        const float* ww, const float* weight, float cell, float w_scale, int G,
+        const float* grid, int apply_weight, float* out)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    return degrid_dev<float>(C, P, support, ov, csh, conv, N, uu, vv, ww, weight, cell,
+            w_scale, G, grid, apply_weight, out);
+}
+
+MI355_EXPORT int mi355_degrid_wproj_d_dev(mi355_ctx* C, size_t P, const int* support,
+        int ov, int csh, const double* conv, size_t N, const double* uu, const double* vv,
+        const double* ww, const double* weight, double cell, double w_scale, int G,
+        const double* grid, int apply_weight, double* out)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    return degrid_dev<double>(C, P, support, ov, csh, conv, N, uu, vv, ww, weight, cell,
+            w_scale, G, grid, apply_weight, out);
+}
+
+MI355_EXPORT int mi355_grid_wproj_f_batched_dev(mi355_ctx* C, size_t P, const int* support,
+        int ov, int csh, const float* conv, int num_jobs, const mi355_job_f* jobs,
+        const float* uu, const float* vv, const float* ww, const float* vis,
+        const float* weight)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    TRY(check_kernel_args(P, support, ov, csh, conv));
+    if (num_jobs < 1 || !jobs) return fail(MI355_ERR_INVALID_ARG, "no jobs");
+    std::vector<JobHost<float, float>> J(num_jobs);
+    unsigned long long n = 0;
+    for (int j = 0; j < num_jobs; ++j)
+    {
+        if (!jobs[j].grid || !jobs[j].norm || !jobs[j].num_skipped)
+            return fail(MI355_ERR_INVALID_ARG, "job %d: NULL output pointer", j);
+        J[j].vis_offset = jobs[j].vis_offset;
+        J[j].num_points = jobs[j].num_points;
+        J[j].cell_size_rad = jobs[j].cell_size_rad;
+        J[j].w_scale = jobs[j].w_scale;
+        J[j].grid_size = jobs[j].grid_size;
+        J[j].grid = reinterpret_cast<float2*>(jobs[j].grid);
+        J[j].norm = jobs[j].norm;
+        J[j].skipped = jobs[j].num_skipped;
+        n += jobs[j].num_points;
+    }
+    if (n && (!uu || !vv || !ww || !vis || !weight))
+        return fail(MI355_ERR_INVALID_ARG, "NULL visibility array");
+    HIP_TRY(hipSetDevice(C->device));
+    TRY((run_pipeline<float, float>(C, P, support, ov, csh, reinterpret_cast<const float2*>(conv),
+            num_jobs, J.data(), uu, vv, ww, vis, weight, nullptr)));
+    collect_timing(C);
+    return ok();
+}
+
+} // extern "C"

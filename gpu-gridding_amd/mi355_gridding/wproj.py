@@ -1,0 +1,98 @@
+"""Host-side mirror of the reference's gridding interface.
+
+``grid_wproj_f`` and ``grid_wproj_d`` take the arguments of
+``oskar_grid_wproj_f`` / ``oskar_grid_wproj_d`` (oskar_grid_wproj.h:68-131), in
+the same order and with the same meaning, as numpy arrays, and call the
+library's drop-in C entry points (mi355_grid_wproj_f / _d).  As in the
+reference, ``grid`` is accumulated in place and ``norm`` is accumulated; the
+C functions return num_skipped and the updated norm through pointers, which
+these wrappers return as ``(num_skipped, norm)``.
+
+Errors the reference would abort on (oskar_grid_wproj_gpu.cu:1833-1843,
+gpu_support.hpp:143-151) raise :class:`GridderError` here.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+def _arr(x, dtype, n=None, name=""):
+    a = np.ascontiguousarray(x, dtype=dtype)
+    if n is not None and a.size != n:
+        raise ValueError("%s: expected %d values, got %d" % (name, n, a.size))
+    return a
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _grid_common(fn, real, num_w_planes, support, oversample, conv_size_half,
+                 conv_func, num_points, uu, vv, ww, vis, weight, cell_size_rad,
+                 w_scale, grid_size, norm, grid):
+    P = int(num_w_planes)
+    N = int(num_points)
+    G = int(grid_size)
+    support = _arr(support, np.int32, P, "support")
+    conv = _arr(conv_func, real, None, "conv_func")
+    if conv.size != P * int(conv_size_half) ** 2 * 2:
+        raise ValueError("conv_func: expected num_w_planes*conv_size_half^2 complex values")
+    uu = _arr(uu, real, N, "uu")
+    vv = _arr(vv, real, N, "vv")
+    ww = _arr(ww, real, N, "ww")
+    vis = _arr(vis, real, 2 * N, "vis")
+    weight = _arr(weight, real, N, "weight")
+    if not isinstance(grid, np.ndarray) or not grid.flags.c_contiguous:
+        raise ValueError("grid must be a C-contiguous numpy array (accumulated in place)")
+    if grid.dtype == np.complex64 and real == np.float32 or \
+            grid.dtype == np.complex128 and real == np.float64:
+        if grid.size != G * G:
+            raise ValueError("grid: expected grid_size^2 complex values")
+    elif grid.dtype == real:
+        if grid.size != 2 * G * G:
+            raise ValueError("grid: expected grid_size^2 complex values")
+    else:
+        raise ValueError("grid dtype %s does not match the precision" % grid.dtype)
+    skipped = ctypes.c_size_t(0)
+    nrm = ctypes.c_double(float(norm))
+    scal = ctypes.c_float if real == np.float32 else ctypes.c_double
+    fn(P, _ptr(support), int(oversample), int(conv_size_half), _ptr(conv), N,
+       _ptr(uu), _ptr(vv), _ptr(ww), _ptr(vis), _ptr(weight),
+       scal(cell_size_rad), scal(w_scale), G, ctypes.byref(skipped),
+       ctypes.byref(nrm), ctypes.c_void_p(grid.ctypes.data))
+    _lib.check_last()
+    return skipped.value, nrm.value
+
+
+def grid_wproj_f(num_w_planes, support, oversample, conv_size_half, conv_func,
+                 num_points, uu, vv, ww, vis, weight, cell_size_rad, w_scale,
+                 grid_size, norm, grid):
+    """FP32 gridding; mirrors oskar_grid_wproj_f (oskar_grid_wproj.h:113-131).
+
+    Returns (num_skipped, norm); ``grid`` (complex64 [G, G] or float32
+    [G, G, 2]) is accumulated in place.
+    """
+    lib = _lib.load()
+    return _grid_common(lib.mi355_grid_wproj_f, np.float32, num_w_planes, support,
+                        oversample, conv_size_half, conv_func, num_points, uu, vv,
+                        ww, vis, weight, cell_size_rad, w_scale, grid_size, norm, grid)
+
+
+def grid_wproj_d(num_w_planes, support, oversample, conv_size_half, conv_func,
+                 num_points, uu, vv, ww, vis, weight, cell_size_rad, w_scale,
+                 grid_size, norm, grid):
+    """FP64 gridding; mirrors oskar_grid_wproj_d (oskar_grid_wproj.h:68-86)."""
+    lib = _lib.load()
+    return _grid_common(lib.mi355_grid_wproj_d, np.float64, num_w_planes, support,
+                        oversample, conv_size_half, conv_func, num_points, uu, vv,
+                        ww, vis, weight, cell_size_rad, w_scale, grid_size, norm, grid)
+
+
+def set_default_option(key, value):
+    """Set an option (_lib.OPT_*) on the context the drop-ins use."""
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    _lib.check(lib.mi355_default_ctx(ctypes.byref(h)))
+    _lib.check(lib.mi355_ctx_set_option(h, int(key), int(value)))

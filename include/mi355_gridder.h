@@ -1,0 +1,221 @@
+/*
+ * mi355_gridder.h -- C ABI of the MI355X (gfx950) W-projection gridder.
+ *
+ * The library replaces the reference's serial convolutional gridder
+ * (oskar_grid_wproj.c) behind the same C signature.  All entry points are
+ * plain C: pointers, sizes and status codes, no C++ or framework types.
+ *
+ * Semantics kept from the serial gridder (oskar_grid_wproj.c:126-212):
+ *   - num_skipped is SET (zeroed, then counted) on every call (:151, :185);
+ *   - norm and grid are ACCUMULATED across calls (:205-206, :210);
+ *   - cell, oversampled offset and w-plane selection are bit-identical to
+ *     the serial code's float (for _f) or double (for _d) arithmetic;
+ *   - the grid is G*G complex values, row-major, re/im interleaved.
+ * The grid itself is summed in a different order (per-tile register
+ * accumulation, then an FP64 merge of tile partials), so FP32 results agree
+ * with the serial code to rounding, not bit for bit.
+ *
+ * Errors: the drop-ins return void, like the functions they replace; the
+ * status of the most recent call on the calling thread is available from
+ * mi355_grid_last_error().  All other entry points return that status.
+ */
+#ifndef MI355_GRIDDER_H_
+#define MI355_GRIDDER_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MI355_EXPORT __attribute__((visibility("default")))
+
+/* Status codes. */
+#define MI355_OK                   0
+#define MI355_ERR_INVALID_ARG      1  /* null pointer, bad size, oversample < 1 ... */
+#define MI355_ERR_KERNEL_TOO_SMALL 2  /* conv_size_half too small for a plane's support */
+#define MI355_ERR_NO_DEVICE        3  /* no HIP device visible */
+#define MI355_ERR_HIP              4  /* a HIP runtime call failed */
+#define MI355_ERR_OUT_OF_MEMORY    5  /* device allocation failed */
+#define MI355_ERR_TOO_LARGE        6  /* a count exceeds the library's 32-bit indexing */
+
+/* ------------------------------------------------------------------------
+ * Drop-in replacements (host pointers, synchronous).
+ * ---------------------------------------------------------------------- */
+
+/* Replaces oskar_grid_wproj_d (oskar_grid_wproj.h:68-86); same arguments,
+ * same meaning.  FP64 throughout, like the function it replaces. */
+MI355_EXPORT void mi355_grid_wproj_d(
+        const size_t num_w_planes, const int* support, const int oversample,
+        const int conv_size_half, const double* conv_func,
+        const size_t num_points, const double* uu, const double* vv,
+        const double* ww, const double* vis, const double* weight,
+        const double cell_size_rad, const double w_scale, const int grid_size,
+        size_t* num_skipped, double* norm, double* grid);
+
+/* Replaces oskar_grid_wproj_f (oskar_grid_wproj.h:113-131), the function
+ * main.cpp's replacement hook calls (main.cpp:288-338); same arguments, same
+ * meaning.  The reference's own GPU entry oskar_grid_wproj_f_gpu
+ * (oskar_grid_wproj_gpu.hpp:92-99) narrows num_points to int, takes double
+ * cell and w_scale, zeroes *norm and assigns the grid; this one does not. */
+MI355_EXPORT void mi355_grid_wproj_f(
+        const size_t num_w_planes, const int* support, const int oversample,
+        const int conv_size_half, const float* conv_func,
+        const size_t num_points, const float* uu, const float* vv,
+        const float* ww, const float* vis, const float* weight,
+        const float cell_size_rad, const float w_scale, const int grid_size,
+        size_t* num_skipped, double* norm, float* grid);
+
+/* Status of the last call made on this thread (MI355_OK when it succeeded),
+ * for the void drop-ins.  The reference's GPU path aborts instead
+ * (gpu_support.hpp:143-151). */
+MI355_EXPORT int mi355_grid_last_error(void);
+MI355_EXPORT const char* mi355_grid_last_error_message(void);
+MI355_EXPORT const char* mi355_grid_error_string(int status);
+
+/* ------------------------------------------------------------------------
+ * Contexts: one per device and caller thread; they own the persistent
+ * device workspace and the stream.  The drop-ins use a library-owned
+ * context on the current device, guarded by a mutex.  No reference
+ * counterpart: the reference allocates and frees per call
+ * (oskar_grid_wproj_gpu.cu:1759-2121).
+ * ---------------------------------------------------------------------- */
+typedef struct mi355_ctx mi355_ctx;
+
+MI355_EXPORT int mi355_ctx_create(int device, mi355_ctx** ctx);
+MI355_EXPORT int mi355_ctx_destroy(mi355_ctx* ctx);
+/* The library-owned context the drop-ins use on the current device (for
+ * setting its options); do not destroy it. */
+MI355_EXPORT int mi355_default_ctx(mi355_ctx** ctx);
+/* Work is enqueued on this HIP stream (a hipStream_t; 0 = null stream). */
+MI355_EXPORT int mi355_ctx_set_stream(mi355_ctx* ctx, void* hip_stream);
+
+/* Tuning and behaviour options (mi355_ctx_set_option). */
+#define MI355_OPT_VISITS_PER_ITEM 1  /* tile visits per work item (default 2048) */
+#define MI355_OPT_COMPENSATED     2  /* 1: TwoSum-compensated LDS accumulation */
+#define MI355_OPT_PHASE_TIMING    3  /* 1: record hipEvents around each phase */
+#define MI355_OPT_CACHE_KERNELS   4  /* 1: reuse prepared kernel tables while the
+                                        conv_func pointer and shape are unchanged */
+MI355_EXPORT int mi355_ctx_set_option(mi355_ctx* ctx, int key, long long value);
+
+/* Figures of the last gridding call on this context. */
+typedef struct
+{
+    double ms_prepare;   /* kernel-table expansion and norm table */
+    double ms_select;    /* per-visibility selection (index stage) */
+    double ms_bucket;    /* visit scan, emit and radix sort by tile */
+    double ms_grid;      /* tile gridding kernel */
+    double ms_merge;     /* ordered FP64 merge of multi-item tiles */
+    double ms_total;     /* all of the above, first event to last */
+    unsigned long long num_points;
+    unsigned long long num_skipped;
+    unsigned long long num_visits;  /* (visibility, tile) pairs */
+    unsigned long long num_items;   /* work items launched */
+    unsigned long long num_taps;    /* sum over gridded visibilities of (2s+1)^2 */
+    unsigned long long num_merge_tiles;
+} mi355_stats;
+
+MI355_EXPORT int mi355_ctx_get_stats(const mi355_ctx* ctx, mi355_stats* stats);
+
+/* ------------------------------------------------------------------------
+ * Device-pointer gridding: every array is in device memory except
+ * `support` (host, num_w_planes ints).  num_skipped (one unsigned long long)
+ * and norm (one double) are device pointers; the call sets *num_skipped and
+ * adds to *norm and to grid.  Same arithmetic as the drop-ins.
+ * ---------------------------------------------------------------------- */
+MI355_EXPORT int mi355_grid_wproj_f_dev(mi355_ctx* ctx,
+        size_t num_w_planes, const int* support, int oversample,
+        int conv_size_half, const float* conv_func, size_t num_points,
+        const float* uu, const float* vv, const float* ww, const float* vis,
+        const float* weight, float cell_size_rad, float w_scale,
+        int grid_size, unsigned long long* num_skipped, double* norm,
+        float* grid);
+
+MI355_EXPORT int mi355_grid_wproj_d_dev(mi355_ctx* ctx,
+        size_t num_w_planes, const int* support, int oversample,
+        int conv_size_half, const double* conv_func, size_t num_points,
+        const double* uu, const double* vv, const double* ww,
+        const double* vis, const double* weight, double cell_size_rad,
+        double w_scale, int grid_size, unsigned long long* num_skipped,
+        double* norm, double* grid);
+
+/* FP32 inputs and FP32 selection (bit-identical cells, offsets and planes to
+ * mi355_grid_wproj_f_dev and oskar_grid_wproj_f), accumulated in FP64 into a
+ * complex128 grid.  No reference counterpart: it is the FP64 truth against
+ * which the FP32 grid's error is stated without counting the cell or plane
+ * flips a pure-double selection (oskar_grid_wproj_d) would make. */
+MI355_EXPORT int mi355_grid_wproj_fsel_d_dev(mi355_ctx* ctx,
+        size_t num_w_planes, const int* support, int oversample,
+        int conv_size_half, const float* conv_func, size_t num_points,
+        const float* uu, const float* vv, const float* ww, const float* vis,
+        const float* weight, float cell_size_rad, float w_scale,
+        int grid_size, unsigned long long* num_skipped, double* norm,
+        double* grid);
+
+/* Selection only: for each visibility, the cell, oversampled offsets,
+ * clamped plane, support, conjugation flag (ww > 0) and skip flag that the
+ * gridder uses -- the per-visibility part of oskar_grid_wproj.c:158-187
+ * (float) / :69-98 (double).  out is int32 [8][num_points] in that field
+ * order (grid_u, grid_v, off_u, off_v, plane, support, conj, skip). */
+MI355_EXPORT int mi355_select_wproj_f_dev(mi355_ctx* ctx,
+        size_t num_w_planes, const int* support, int oversample,
+        size_t num_points, const float* uu, const float* vv, const float* ww,
+        float cell_size_rad, float w_scale, int grid_size, int* out);
+MI355_EXPORT int mi355_select_wproj_d_dev(mi355_ctx* ctx,
+        size_t num_w_planes, const int* support, int oversample,
+        size_t num_points, const double* uu, const double* vv, const double* ww,
+        double cell_size_rad, double w_scale, int grid_size, int* out);
+
+/* ------------------------------------------------------------------------
+ * Degridding: the exact adjoint of gridding with respect to the complex
+ * amplitudes (no reference counterpart).  For every visibility i,
+ *   vis_out[i] = w_i * sum over its footprint of conj(c'_i) * grid[cell],
+ * c' being the kernel tap after the conjugation flip for ww > 0; with
+ * apply_weight = 0 the factor w_i is omitted.  Skipped visibilities get 0.
+ * Device pointers except `support`.
+ * ---------------------------------------------------------------------- */
+MI355_EXPORT int mi355_degrid_wproj_f_dev(mi355_ctx* ctx,
+        size_t num_w_planes, const int* support, int oversample,
+        int conv_size_half, const float* conv_func, size_t num_points,
+        const float* uu, const float* vv, const float* ww,
+        const float* weight, float cell_size_rad, float w_scale,
+        int grid_size, const float* grid, int apply_weight, float* vis_out);
+
+MI355_EXPORT int mi355_degrid_wproj_d_dev(mi355_ctx* ctx,
+        size_t num_w_planes, const int* support, int oversample,
+        int conv_size_half, const double* conv_func, size_t num_points,
+        const double* uu, const double* vv, const double* ww,
+        const double* weight, double cell_size_rad, double w_scale,
+        int grid_size, const double* grid, int apply_weight,
+        double* vis_out);
+
+/* ------------------------------------------------------------------------
+ * Batched gridding: many independent grids (snapshots / channels) sharing
+ * one kernel stack, gridded in one pass of the pipeline.  Job j grids
+ * visibilities [vis_offset, vis_offset + num_points) of the concatenated
+ * arrays onto its own grid.  The reference grids one block per call
+ * (main.cpp:243-381); this is the same operation for many blocks at once.
+ * ---------------------------------------------------------------------- */
+typedef struct
+{
+    size_t vis_offset;
+    size_t num_points;
+    float cell_size_rad;
+    float w_scale;
+    int grid_size;
+    float* grid;                      /* device, grid_size^2 complex */
+    double* norm;                     /* device, accumulated */
+    unsigned long long* num_skipped;  /* device, set */
+} mi355_job_f;
+
+MI355_EXPORT int mi355_grid_wproj_f_batched_dev(mi355_ctx* ctx,
+        size_t num_w_planes, const int* support, int oversample,
+        int conv_size_half, const float* conv_func, int num_jobs,
+        const mi355_job_f* jobs, const float* uu, const float* vv,
+        const float* ww, const float* vis, const float* weight);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MI355_GRIDDER_H_ */

@@ -1,0 +1,443 @@
+/*
+ * grid_oracle.c -- CPU restatement of the serial W-projection gridder.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this file; only tests/, the smoke check and the CPU-baseline leg of
+ * bench.py use it, and only as the checker.
+ *
+ * It restates, function by function, the reference's single-threaded
+ * gridders:
+ *   oskar_grid_wproj_f   oskar_grid_wproj.c:126-212  (FP32)
+ *   oskar_grid_wproj_d   oskar_grid_wproj.c:37-123   (FP64)
+ * keeping every floating-point operation, its precision and its order, so
+ * that on x86-64 (gcc, -O2 -ffp-contract=off, no FMA) the grid, norm and
+ * num_skipped are bit-identical to the compiled reference.  That claim is
+ * pinned by tests/test_oracle.py against (1) the committed golden fixtures in
+ * tests/golden/, written by the compiled reference, (2) oracle/_ref/ (the
+ * reference compiled from its own sources) when present, and (3) closed forms
+ * (delta and constant kernels, conjugation symmetry, linearity, Hermitian
+ * mirror, norm table, brute-force cell selection on tiny grids).
+ *
+ * Additional checkers with no reference counterpart (pinned by the same
+ * closed forms and by agreement with the functions above where they coincide):
+ *   oracle_select_f / _d       per-visibility cell / offset / plane selection
+ *                              (the index part of oskar_grid_wproj.c:158-187)
+ *   oracle_grid_wproj_fsel_d   FP32 selection, FP64 accumulation: the FP64
+ *                              truth that the FP32 GPU grid is measured
+ *                              against without counting selection flips.
+ *   oracle_degrid_wproj_fsel_d exact adjoint of the above (FP64).
+ *   oracle_grid_wproj_fsel_d_tiles  the same FP64 truth restricted to a list
+ *                              of 64x64 grid tiles, multi-threaded, for
+ *                              spot checks of full-size grids.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <pthread.h>
+
+#define ORACLE_API __attribute__((visibility("default")))
+
+/* Selection of one visibility: cell, oversampled offsets, plane and support.
+ * Mirrors oskar_grid_wproj.c:158-187 (float) and :69-98 (double). */
+typedef struct
+{
+    int grid_u, grid_v;     /* nearest cell (oskar_grid_wproj.c:163-164) */
+    int off_u, off_v;       /* oversampled offsets (:172-173) */
+    size_t plane;           /* plane actually used, after the clamp (:176-179) */
+    int support;            /* w_support (:176-177) */
+    int conj_neg;           /* 1 when ww > 0, i.e. Im(kernel) is negated (:161) */
+    int skip;               /* footprint leaves the grid (:182-187) */
+} oracle_sel;
+
+static void select_f(float u, float v, float w, float grid_scale,
+        float w_scale, int grid_size, int oversample, size_t num_w_planes,
+        const int* support, oracle_sel* s)
+{
+    const int centre = grid_size / 2;
+    const float pos_u = -u * grid_scale;
+    const float pos_v = v * grid_scale;
+    const size_t wi = (size_t)roundf(sqrtf(fabsf(w * w_scale)));
+    const float ru = roundf(pos_u), rv = roundf(pos_v);
+    s->conj_neg = (w > 0.0f);
+    s->grid_u = (int)ru + centre;
+    s->grid_v = (int)rv + centre;
+    s->off_u = (int)roundf((ru - pos_u) * oversample);
+    s->off_v = (int)roundf((rv - pos_v) * oversample);
+    s->plane = wi < num_w_planes ? wi : num_w_planes - 1;
+    s->support = support[s->plane];
+    s->skip = (s->grid_u + s->support >= grid_size ||
+            s->grid_u - s->support < 0 ||
+            s->grid_v + s->support >= grid_size ||
+            s->grid_v - s->support < 0);
+}
+
+static void select_d(double u, double v, double w, double grid_scale,
+        double w_scale, int grid_size, int oversample, size_t num_w_planes,
+        const int* support, oracle_sel* s)
+{
+    const int centre = grid_size / 2;
+    const double pos_u = -u * grid_scale;
+    const double pos_v = v * grid_scale;
+    const size_t wi = (size_t)round(sqrt(fabs(w * w_scale)));
+    const double ru = round(pos_u), rv = round(pos_v);
+    s->conj_neg = (w > 0.0);
+    s->grid_u = (int)ru + centre;
+    s->grid_v = (int)rv + centre;
+    s->off_u = (int)round((ru - pos_u) * oversample);
+    s->off_v = (int)round((rv - pos_v) * oversample);
+    s->plane = wi < num_w_planes ? wi : num_w_planes - 1;
+    s->support = support[s->plane];
+    s->skip = (s->grid_u + s->support >= grid_size ||
+            s->grid_u - s->support < 0 ||
+            s->grid_v + s->support >= grid_size ||
+            s->grid_v - s->support < 0);
+}
+
+static void unpack(const oracle_sel* s, size_t i, int* gu, int* gv, int* ou,
+        int* ov, int* plane, int* sup, int* conj_neg, int* skip)
+{
+    if (gu) gu[i] = s->grid_u;
+    if (gv) gv[i] = s->grid_v;
+    if (ou) ou[i] = s->off_u;
+    if (ov) ov[i] = s->off_v;
+    if (plane) plane[i] = (int)s->plane;
+    if (sup) sup[i] = s->support;
+    if (conj_neg) conj_neg[i] = s->conj_neg;
+    if (skip) skip[i] = s->skip;
+}
+
+ORACLE_API void oracle_select_f(size_t n, const float* uu, const float* vv,
+        const float* ww, size_t num_w_planes, const int* support,
+        int oversample, float cell_size_rad, float w_scale, int grid_size,
+        int* gu, int* gv, int* ou, int* ov, int* plane, int* sup,
+        int* conj_neg, int* skip)
+{
+    const float grid_scale = grid_size * cell_size_rad; /* :148 */
+    size_t i;
+    for (i = 0; i < n; ++i)
+    {
+        oracle_sel s;
+        select_f(uu[i], vv[i], ww[i], grid_scale, w_scale, grid_size,
+                oversample, num_w_planes, support, &s);
+        unpack(&s, i, gu, gv, ou, ov, plane, sup, conj_neg, skip);
+    }
+}
+
+ORACLE_API void oracle_select_d(size_t n, const double* uu, const double* vv,
+        const double* ww, size_t num_w_planes, const int* support,
+        int oversample, double cell_size_rad, double w_scale, int grid_size,
+        int* gu, int* gv, int* ou, int* ov, int* plane, int* sup,
+        int* conj_neg, int* skip)
+{
+    const double grid_scale = grid_size * cell_size_rad; /* :59 */
+    size_t i;
+    for (i = 0; i < n; ++i)
+    {
+        oracle_sel s;
+        select_d(uu[i], vv[i], ww[i], grid_scale, w_scale, grid_size,
+                oversample, num_w_planes, support, &s);
+        unpack(&s, i, gu, gv, ou, ov, plane, sup, conj_neg, skip);
+    }
+}
+
+/* FP32 gridder, same signature and semantics as oskar_grid_wproj_f
+ * (oskar_grid_wproj.h:113-131): num_skipped is set, norm and grid are
+ * accumulated.  The footprint walk follows oskar_grid_wproj.c:190-210: rows
+ * j outer, columns k inner, kernel tap at |off_v+j*ov| x |off_u+k*ov|,
+ * complex MAC in float, sum of Re(tap) in double. */
+ORACLE_API void oracle_grid_wproj_f(size_t num_w_planes, const int* support,
+        int oversample, int conv_size_half, const float* conv_func,
+        size_t num_points, const float* uu, const float* vv, const float* ww,
+        const float* vis, const float* weight, float cell_size_rad,
+        float w_scale, int grid_size, size_t* num_skipped, double* norm,
+        float* grid)
+{
+    const size_t plane_len = (size_t)conv_size_half * conv_size_half;
+    const float grid_scale = grid_size * cell_size_rad;
+    size_t i;
+    *num_skipped = 0;
+    for (i = 0; i < num_points; ++i)
+    {
+        oracle_sel s;
+        int j, k;
+        double tap_sum = 0.0;
+        select_f(uu[i], vv[i], ww[i], grid_scale, w_scale, grid_size,
+                oversample, num_w_planes, support, &s);
+        if (s.skip) { *num_skipped += 1; continue; }
+        {
+            const float w_i = weight[i];
+            const float a_re = w_i * vis[2 * i];
+            const float a_im = w_i * vis[2 * i + 1];
+            const float cj = s.conj_neg ? -1.0f : 1.0f;
+            const float* plane = conv_func + 2 * s.plane * plane_len;
+            for (j = -s.support; j <= s.support; ++j)
+            {
+                const float* krow = plane +
+                        2 * (size_t)abs(s.off_v + j * oversample) * conv_size_half;
+                float* grow = grid + 2 * ((size_t)(s.grid_v + j) * grid_size
+                        + s.grid_u);
+                for (k = -s.support; k <= s.support; ++k)
+                {
+                    const float* c = krow + 2 * abs(s.off_u + k * oversample);
+                    const float c_re = c[0];
+                    const float c_im = c[1] * cj;
+                    grow[2 * k]     += (a_re * c_re - a_im * c_im);
+                    grow[2 * k + 1] += (a_im * c_re + a_re * c_im);
+                    tap_sum += c_re;
+                }
+            }
+            *norm += tap_sum * w_i;
+        }
+    }
+}
+
+/* FP64 gridder, oskar_grid_wproj_d (oskar_grid_wproj.h:68-86,
+ * oskar_grid_wproj.c:37-123). */
+ORACLE_API void oracle_grid_wproj_d(size_t num_w_planes, const int* support,
+        int oversample, int conv_size_half, const double* conv_func,
+        size_t num_points, const double* uu, const double* vv,
+        const double* ww, const double* vis, const double* weight,
+        double cell_size_rad, double w_scale, int grid_size,
+        size_t* num_skipped, double* norm, double* grid)
+{
+    const size_t plane_len = (size_t)conv_size_half * conv_size_half;
+    const double grid_scale = grid_size * cell_size_rad;
+    size_t i;
+    *num_skipped = 0;
+    for (i = 0; i < num_points; ++i)
+    {
+        oracle_sel s;
+        int j, k;
+        double tap_sum = 0.0;
+        select_d(uu[i], vv[i], ww[i], grid_scale, w_scale, grid_size,
+                oversample, num_w_planes, support, &s);
+        if (s.skip) { *num_skipped += 1; continue; }
+        {
+            const double w_i = weight[i];
+            const double a_re = w_i * vis[2 * i];
+            const double a_im = w_i * vis[2 * i + 1];
+            const double cj = s.conj_neg ? -1.0 : 1.0;
+            const double* plane = conv_func + 2 * s.plane * plane_len;
+            for (j = -s.support; j <= s.support; ++j)
+            {
+                const double* krow = plane +
+                        2 * (size_t)abs(s.off_v + j * oversample) * conv_size_half;
+                double* grow = grid + 2 * ((size_t)(s.grid_v + j) * grid_size
+                        + s.grid_u);
+                for (k = -s.support; k <= s.support; ++k)
+                {
+                    const double* c = krow + 2 * abs(s.off_u + k * oversample);
+                    const double c_re = c[0];
+                    const double c_im = c[1] * cj;
+                    grow[2 * k]     += (a_re * c_re - a_im * c_im);
+                    grow[2 * k + 1] += (a_im * c_re + a_re * c_im);
+                    tap_sum += c_re;
+                }
+            }
+            *norm += tap_sum * w_i;
+        }
+    }
+}
+
+/* FP64 truth for the FP32 gridder: selection exactly as oskar_grid_wproj_f,
+ * accumulation of the same (float) inputs in double.  No reference
+ * counterpart; it coincides with oskar_grid_wproj_d wherever the float and
+ * double selections agree. */
+ORACLE_API void oracle_grid_wproj_fsel_d(size_t num_w_planes,
+        const int* support, int oversample, int conv_size_half,
+        const float* conv_func, size_t num_points, const float* uu,
+        const float* vv, const float* ww, const float* vis,
+        const float* weight, float cell_size_rad, float w_scale,
+        int grid_size, size_t* num_skipped, double* norm, double* grid)
+{
+    const size_t plane_len = (size_t)conv_size_half * conv_size_half;
+    const float grid_scale = grid_size * cell_size_rad;
+    size_t i;
+    *num_skipped = 0;
+    for (i = 0; i < num_points; ++i)
+    {
+        oracle_sel s;
+        int j, k;
+        double tap_sum = 0.0;
+        select_f(uu[i], vv[i], ww[i], grid_scale, w_scale, grid_size,
+                oversample, num_w_planes, support, &s);
+        if (s.skip) { *num_skipped += 1; continue; }
+        {
+            const double w_i = weight[i];
+            const double a_re = w_i * (double)vis[2 * i];
+            const double a_im = w_i * (double)vis[2 * i + 1];
+            const double cj = s.conj_neg ? -1.0 : 1.0;
+            const float* plane = conv_func + 2 * s.plane * plane_len;
+            for (j = -s.support; j <= s.support; ++j)
+            {
+                const float* krow = plane +
+                        2 * (size_t)abs(s.off_v + j * oversample) * conv_size_half;
+                double* grow = grid + 2 * ((size_t)(s.grid_v + j) * grid_size
+                        + s.grid_u);
+                for (k = -s.support; k <= s.support; ++k)
+                {
+                    const float* c = krow + 2 * abs(s.off_u + k * oversample);
+                    const double c_re = c[0];
+                    const double c_im = (double)c[1] * cj;
+                    grow[2 * k]     += (a_re * c_re - a_im * c_im);
+                    grow[2 * k + 1] += (a_im * c_re + a_re * c_im);
+                    tap_sum += c_re;
+                }
+            }
+            *norm += tap_sum * w_i;
+        }
+    }
+}
+
+/* Exact adjoint (FP64) of oracle_grid_wproj_fsel_d with respect to the
+ * complex amplitudes:  out[i] = w_i * sum_taps conj(c'_i) * grid[cell],
+ * where c' is the tap after the conjugation flip.  Skipped visibilities get 0.
+ * With apply_weight = 0 the factor w_i is left out. */
+ORACLE_API void oracle_degrid_wproj_fsel_d(size_t num_w_planes,
+        const int* support, int oversample, int conv_size_half,
+        const float* conv_func, size_t num_points, const float* uu,
+        const float* vv, const float* ww, const float* weight,
+        float cell_size_rad, float w_scale, int grid_size,
+        const double* grid, int apply_weight, double* out)
+{
+    const size_t plane_len = (size_t)conv_size_half * conv_size_half;
+    const float grid_scale = grid_size * cell_size_rad;
+    size_t i;
+    for (i = 0; i < num_points; ++i)
+    {
+        oracle_sel s;
+        int j, k;
+        double acc_re = 0.0, acc_im = 0.0;
+        select_f(uu[i], vv[i], ww[i], grid_scale, w_scale, grid_size,
+                oversample, num_w_planes, support, &s);
+        out[2 * i] = 0.0;
+        out[2 * i + 1] = 0.0;
+        if (s.skip) continue;
+        {
+            const double cj = s.conj_neg ? -1.0 : 1.0;
+            const float* plane = conv_func + 2 * s.plane * plane_len;
+            const double w_i = apply_weight ? (double)weight[i] : 1.0;
+            for (j = -s.support; j <= s.support; ++j)
+            {
+                const float* krow = plane +
+                        2 * (size_t)abs(s.off_v + j * oversample) * conv_size_half;
+                const double* grow = grid + 2 * ((size_t)(s.grid_v + j) *
+                        grid_size + s.grid_u);
+                for (k = -s.support; k <= s.support; ++k)
+                {
+                    const float* c = krow + 2 * abs(s.off_u + k * oversample);
+                    const double c_re = c[0];
+                    const double c_im = (double)c[1] * cj;
+                    const double g_re = grow[2 * k], g_im = grow[2 * k + 1];
+                    acc_re += c_re * g_re + c_im * g_im;
+                    acc_im += c_re * g_im - c_im * g_re;
+                }
+            }
+            out[2 * i] = w_i * acc_re;
+            out[2 * i + 1] = w_i * acc_im;
+        }
+    }
+}
+
+/* ---- Tile-restricted FP64 truth for spot checks of large grids. ---- */
+
+typedef struct
+{
+    size_t num_w_planes; const int* support; int oversample;
+    int conv_size_half; const float* conv_func; size_t num_points;
+    const float* uu; const float* vv; const float* ww; const float* vis;
+    const float* weight; float cell_size_rad; float w_scale; int grid_size;
+    int tile; const int* tiles_xy; size_t num_tiles; double* out;
+    size_t next; pthread_mutex_t lock;
+} tile_job;
+
+static void* tile_worker(void* arg)
+{
+    tile_job* J = (tile_job*)arg;
+    const size_t plane_len = (size_t)J->conv_size_half * J->conv_size_half;
+    const float grid_scale = J->grid_size * J->cell_size_rad;
+    const int T = J->tile;
+    for (;;)
+    {
+        size_t t, i;
+        pthread_mutex_lock(&J->lock);
+        t = J->next++;
+        pthread_mutex_unlock(&J->lock);
+        if (t >= J->num_tiles) break;
+        {
+            const int x0 = J->tiles_xy[2 * t] * T, y0 = J->tiles_xy[2 * t + 1] * T;
+            double* o = J->out + 2 * (size_t)T * T * t;
+            memset(o, 0, sizeof(double) * 2 * (size_t)T * T);
+            for (i = 0; i < J->num_points; ++i)
+            {
+                oracle_sel s;
+                int j, k, jlo, jhi, klo, khi;
+                select_f(J->uu[i], J->vv[i], J->ww[i], grid_scale, J->w_scale,
+                        J->grid_size, J->oversample, J->num_w_planes,
+                        J->support, &s);
+                if (s.skip) continue;
+                jlo = y0 - s.grid_v; if (jlo < -s.support) jlo = -s.support;
+                jhi = y0 + T - 1 - s.grid_v; if (jhi > s.support) jhi = s.support;
+                klo = x0 - s.grid_u; if (klo < -s.support) klo = -s.support;
+                khi = x0 + T - 1 - s.grid_u; if (khi > s.support) khi = s.support;
+                if (jlo > jhi || klo > khi) continue;
+                {
+                    const double w_i = J->weight[i];
+                    const double a_re = w_i * (double)J->vis[2 * i];
+                    const double a_im = w_i * (double)J->vis[2 * i + 1];
+                    const double cj = s.conj_neg ? -1.0 : 1.0;
+                    const float* plane = J->conv_func + 2 * s.plane * plane_len;
+                    for (j = jlo; j <= jhi; ++j)
+                    {
+                        const float* krow = plane + 2 * (size_t)abs(
+                                s.off_v + j * J->oversample) * J->conv_size_half;
+                        double* orow = o + 2 * (size_t)(s.grid_v + j - y0) * T;
+                        for (k = klo; k <= khi; ++k)
+                        {
+                            const float* c = krow + 2 * abs(s.off_u + k * J->oversample);
+                            const double c_re = c[0];
+                            const double c_im = (double)c[1] * cj;
+                            const int x = s.grid_u + k - x0;
+                            orow[2 * x]     += (a_re * c_re - a_im * c_im);
+                            orow[2 * x + 1] += (a_im * c_re + a_re * c_im);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    return NULL;
+}
+
+/* FP64 truth (as oracle_grid_wproj_fsel_d) for the listed tiles only:
+ * tiles_xy holds (tile_x, tile_y) pairs of tile x tile cells; out receives
+ * num_tiles blocks of tile*tile complex doubles, row-major.  Contributions
+ * only (the block starts at zero).  Uses num_threads pthreads. */
+ORACLE_API void oracle_grid_wproj_fsel_d_tiles(size_t num_w_planes,
+        const int* support, int oversample, int conv_size_half,
+        const float* conv_func, size_t num_points, const float* uu,
+        const float* vv, const float* ww, const float* vis,
+        const float* weight, float cell_size_rad, float w_scale,
+        int grid_size, int tile, const int* tiles_xy, size_t num_tiles,
+        int num_threads, double* out)
+{
+    tile_job J;
+    pthread_t th[256];
+    int t;
+    J.num_w_planes = num_w_planes; J.support = support;
+    J.oversample = oversample; J.conv_size_half = conv_size_half;
+    J.conv_func = conv_func; J.num_points = num_points;
+    J.uu = uu; J.vv = vv; J.ww = ww; J.vis = vis; J.weight = weight;
+    J.cell_size_rad = cell_size_rad; J.w_scale = w_scale;
+    J.grid_size = grid_size; J.tile = tile; J.tiles_xy = tiles_xy;
+    J.num_tiles = num_tiles; J.out = out; J.next = 0;
+    pthread_mutex_init(&J.lock, NULL);
+    if (num_threads < 1) num_threads = 1;
+    if (num_threads > 256) num_threads = 256;
+    for (t = 0; t < num_threads; ++t)
+        pthread_create(&th[t], NULL, tile_worker, &J);
+    for (t = 0; t < num_threads; ++t)
+        pthread_join(th[t], NULL);
+    pthread_mutex_destroy(&J.lock);
+}

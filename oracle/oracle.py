@@ -1,0 +1,193 @@
+"""ctypes access to the CPU checkers -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this
+module, as the checker; the product library never does.
+
+  liboracle.so   C restatement of oskar_grid_wproj_f/_d (grid_oracle.c)
+  _ref/libref.so the reference's own oskar_grid_wproj.c compiled unmodified
+                 (built only where the reference tree exists; optional)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libref.so")
+
+P = ctypes.c_void_p
+SZ = ctypes.c_size_t
+I = ctypes.c_int
+F = ctypes.c_float
+D = ctypes.c_double
+
+_GRID_F = [SZ, P, I, I, P, SZ, P, P, P, P, P, F, F, I, P, P, P]
+_GRID_D = [SZ, P, I, I, P, SZ, P, P, P, P, P, D, D, I, P, P, P]
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise RuntimeError("oracle not built: make -C oracle")
+        lib = ctypes.CDLL(ORACLE_PATH)
+        lib.oracle_grid_wproj_f.argtypes = _GRID_F
+        lib.oracle_grid_wproj_d.argtypes = _GRID_D
+        lib.oracle_grid_wproj_fsel_d.argtypes = _GRID_F
+        lib.oracle_select_f.argtypes = [SZ, P, P, P, SZ, P, I, F, F, I] + [P] * 8
+        lib.oracle_select_d.argtypes = [SZ, P, P, P, SZ, P, I, D, D, I] + [P] * 8
+        lib.oracle_degrid_wproj_fsel_d.argtypes = [SZ, P, I, I, P, SZ, P, P, P, P, F, F, I,
+                                                   P, I, P]
+        lib.oracle_grid_wproj_fsel_d_tiles.argtypes = [SZ, P, I, I, P, SZ, P, P, P, P, P, F, F,
+                                                       I, I, P, SZ, I, P]
+        for fn in (lib.oracle_grid_wproj_f, lib.oracle_grid_wproj_d, lib.oracle_grid_wproj_fsel_d,
+                   lib.oracle_select_f, lib.oracle_select_d, lib.oracle_degrid_wproj_fsel_d,
+                   lib.oracle_grid_wproj_fsel_d_tiles):
+            fn.restype = None
+        _oracle = lib
+    return _oracle
+
+
+def ref():
+    """The compiled reference, or None where it was not built."""
+    global _ref
+    if _ref is None and os.path.exists(REF_PATH):
+        lib = ctypes.CDLL(REF_PATH)
+        lib.oskar_grid_wproj_f.argtypes = _GRID_F
+        lib.oskar_grid_wproj_d.argtypes = _GRID_D
+        lib.oskar_grid_wproj_f.restype = None
+        lib.oskar_grid_wproj_d.restype = None
+        lib.check_value_float.argtypes = [F, F]
+        lib.check_value_float.restype = I
+        _ref = lib
+    return _ref
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _inputs(case, real):
+    c = case
+    return dict(
+        support=np.ascontiguousarray(c["support"], np.int32),
+        conv=np.ascontiguousarray(c["conv_func"], real),
+        uu=np.ascontiguousarray(c["uu"], real),
+        vv=np.ascontiguousarray(c["vv"], real),
+        ww=np.ascontiguousarray(c["ww"], real),
+        vis=np.ascontiguousarray(c["vis"], real),
+        weight=np.ascontiguousarray(c["weight"], real),
+    )
+
+
+def _call(fn, case, real, grid_real, scal, grid=None, norm=0.0):
+    x = _inputs(case, real)
+    G = int(case["grid_size"])
+    N = int(case["num_points"])
+    if grid is None:
+        grid = np.zeros((G, G, 2), grid_real)
+    skipped = ctypes.c_size_t(0)
+    nrm = ctypes.c_double(norm)
+    fn(int(case["num_w_planes"]), _p(x["support"]), int(case["oversample"]),
+       int(case["conv_size_half"]), _p(x["conv"]), N, _p(x["uu"]), _p(x["vv"]), _p(x["ww"]),
+       _p(x["vis"]), _p(x["weight"]), scal(float(case["cell_size_rad"])),
+       scal(float(case["w_scale"])), G, ctypes.byref(skipped), ctypes.byref(nrm), _p(grid))
+    return grid, nrm.value, skipped.value
+
+
+def grid_f(case, use_ref=False, grid=None, norm=0.0):
+    """oskar_grid_wproj_f semantics -> (grid float32 [G,G,2], norm, num_skipped)."""
+    lib = ref() if use_ref else oracle()
+    fn = lib.oskar_grid_wproj_f if use_ref else lib.oracle_grid_wproj_f
+    return _call(fn, case, np.float32, np.float32, ctypes.c_float, grid, norm)
+
+
+def grid_d(case, use_ref=False, grid=None, norm=0.0):
+    """oskar_grid_wproj_d semantics on the case's values promoted to double."""
+    lib = ref() if use_ref else oracle()
+    fn = lib.oskar_grid_wproj_d if use_ref else lib.oracle_grid_wproj_d
+    return _call(fn, case, np.float64, np.float64, ctypes.c_double, grid, norm)
+
+
+def grid_fsel_d(case, grid=None, norm=0.0):
+    """FP32 selection, FP64 accumulation (the FP64 truth for FP32 gridding)."""
+    lib = oracle()
+    x = _inputs(case, np.float32)
+    G = int(case["grid_size"])
+    if grid is None:
+        grid = np.zeros((G, G, 2), np.float64)
+    skipped = ctypes.c_size_t(0)
+    nrm = ctypes.c_double(norm)
+    lib.oracle_grid_wproj_fsel_d(
+        int(case["num_w_planes"]), _p(x["support"]), int(case["oversample"]),
+        int(case["conv_size_half"]), _p(x["conv"]), int(case["num_points"]), _p(x["uu"]),
+        _p(x["vv"]), _p(x["ww"]), _p(x["vis"]), _p(x["weight"]),
+        ctypes.c_float(float(case["cell_size_rad"])), ctypes.c_float(float(case["w_scale"])),
+        G, ctypes.byref(skipped), ctypes.byref(nrm), _p(grid))
+    return grid, nrm.value, skipped.value
+
+
+def select_f(case):
+    """Per-visibility selection of oskar_grid_wproj_f: dict of int32 arrays."""
+    lib = oracle()
+    x = _inputs(case, np.float32)
+    N = int(case["num_points"])
+    out = {k: np.zeros(N, np.int32) for k in
+           ("grid_u", "grid_v", "off_u", "off_v", "plane", "support", "conj", "skip")}
+    lib.oracle_select_f(N, _p(x["uu"]), _p(x["vv"]), _p(x["ww"]), int(case["num_w_planes"]),
+                        _p(x["support"]), int(case["oversample"]),
+                        ctypes.c_float(float(case["cell_size_rad"])),
+                        ctypes.c_float(float(case["w_scale"])), int(case["grid_size"]),
+                        *[_p(out[k]) for k in ("grid_u", "grid_v", "off_u", "off_v", "plane",
+                                               "support", "conj", "skip")])
+    return out
+
+
+def select_d(case):
+    lib = oracle()
+    x = _inputs(case, np.float64)
+    N = int(case["num_points"])
+    out = {k: np.zeros(N, np.int32) for k in
+           ("grid_u", "grid_v", "off_u", "off_v", "plane", "support", "conj", "skip")}
+    lib.oracle_select_d(N, _p(x["uu"]), _p(x["vv"]), _p(x["ww"]), int(case["num_w_planes"]),
+                        _p(x["support"]), int(case["oversample"]),
+                        ctypes.c_double(float(case["cell_size_rad"])),
+                        ctypes.c_double(float(case["w_scale"])), int(case["grid_size"]),
+                        *[_p(out[k]) for k in ("grid_u", "grid_v", "off_u", "off_v", "plane",
+                                               "support", "conj", "skip")])
+    return out
+
+
+def degrid_fsel_d(case, grid, apply_weight=True):
+    """FP64 adjoint of grid_fsel_d: out [N, 2] float64."""
+    lib = oracle()
+    x = _inputs(case, np.float32)
+    N = int(case["num_points"])
+    g = np.ascontiguousarray(grid, np.float64)
+    out = np.zeros((N, 2), np.float64)
+    lib.oracle_degrid_wproj_fsel_d(
+        int(case["num_w_planes"]), _p(x["support"]), int(case["oversample"]),
+        int(case["conv_size_half"]), _p(x["conv"]), N, _p(x["uu"]), _p(x["vv"]), _p(x["ww"]),
+        _p(x["weight"]), ctypes.c_float(float(case["cell_size_rad"])),
+        ctypes.c_float(float(case["w_scale"])), int(case["grid_size"]), _p(g),
+        1 if apply_weight else 0, _p(out))
+    return out
+
+
+def tiles_fsel_d(case, tiles_xy, tile=64, threads=8):
+    """FP64 truth on selected tiles: [n, tile, tile, 2] float64."""
+    lib = oracle()
+    x = _inputs(case, np.float32)
+    t = np.ascontiguousarray(tiles_xy, np.int32).reshape(-1, 2)
+    out = np.zeros((t.shape[0], tile, tile, 2), np.float64)
+    lib.oracle_grid_wproj_fsel_d_tiles(
+        int(case["num_w_planes"]), _p(x["support"]), int(case["oversample"]),
+        int(case["conv_size_half"]), _p(x["conv"]), int(case["num_points"]), _p(x["uu"]),
+        _p(x["vv"]), _p(x["ww"]), _p(x["vis"]), _p(x["weight"]),
+        ctypes.c_float(float(case["cell_size_rad"])), ctypes.c_float(float(case["w_scale"])),
+        int(case["grid_size"]), int(tile), _p(t), t.shape[0], int(threads), _p(out))
+    return out

@@ -1,0 +1,72 @@
+"""Shared test set-up.
+
+Markers: ``gpu`` tests need an MI355X (they run the HIP path through the C
+ABI); everything else runs on the CPU.  The CPU checkers come from oracle/
+(see oracle/oracle.py); golden fixtures live in tests/golden/.
+"""
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "gpu-gridding_amd")
+for p in (ROOT, PKG):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU (HIP path)")
+    config.addinivalue_line("markers", "slow: full-size workload")
+
+
+def golden_names():
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+
+
+def load_golden(name):
+    with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:
+        d = {k: z[k] for k in z.files}
+    for k in ("num_w_planes", "oversample", "conv_size_half", "grid_size", "num_points"):
+        d[k] = int(d[k])
+    d["cell_size_rad"] = np.float32(d["cell_size_rad"])
+    d["w_scale"] = np.float32(d["w_scale"])
+    return d
+
+
+def errors(test, truth):
+    """Norm-wise errors of a complex grid ([..., 2] real or complex arrays):
+    max|a-b| / max|b| and ||a-b||_2 / ||b||_2 (per-cell relative error is
+    ill-posed where the grid cancels)."""
+    a = np.asarray(test, np.float64)
+    b = np.asarray(truth, np.float64)
+    d = a - b
+    bmax = np.max(np.abs(b))
+    return (float(np.max(np.abs(d)) / bmax) if bmax > 0 else float(np.max(np.abs(d))),
+            float(np.linalg.norm(d) / np.linalg.norm(b)) if np.linalg.norm(b) > 0
+            else float(np.linalg.norm(d)))
+
+
+def same_bits(a, b):
+    """Bitwise equality, NaN-aware."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    return a.shape == b.shape and a.dtype == b.dtype and a.tobytes() == b.tobytes()
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    """torch + the gridder library on cuda:0; fails (does not skip) when the
+    library is missing, skips only where no GPU is visible."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU visible")
+    from mi355_gridding import _lib
+    _lib.load()
+    from mi355_gridding import torch_ops
+    return torch, torch_ops
