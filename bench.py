@@ -73,6 +73,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--kernel", choices=("staged", "direct"), default="staged",
+                    help="tile kernel: kernel tables staged in LDS, or read directly")
+    ap.add_argument("--visits-per-item", type=int, default=2048)
+    ap.add_argument("--min-stage-run", type=int, default=2)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (the gridder has no multi-GPU "
@@ -100,6 +104,9 @@ def main():
     ctx = ops.context()
     ctx.set_option(_lib.OPT_PHASE_TIMING, 1)
     ctx.set_option(_lib.OPT_COMPENSATED, 1 if a.compensated else 0)
+    ctx.set_option(_lib.OPT_GRID_KERNEL, 0 if a.kernel == "staged" else 1)
+    ctx.set_option(_lib.OPT_VISITS_PER_ITEM, a.visits_per_item)
+    ctx.set_option(_lib.OPT_MIN_STAGE_RUN, a.min_stage_run)
 
     def step():
         ops.grid(ks, d["uu"], d["vv"], d["ww"], d["vis"], d["weight"], case["cell_size_rad"],
@@ -180,7 +187,8 @@ def main():
         "config": {"workload": case["name"], "num_points": N, "grid_size": G,
                    "num_w_planes": case["num_w_planes"], "max_support": case["max_support"],
                    "oversample": case["oversample"],
-                   "accumulation": "compensated" if a.compensated else "plain"},
+                   "accumulation": "compensated" if a.compensated else "plain",
+                   "tile_kernel": a.kernel, "visits_per_item": a.visits_per_item},
         "roofline": {"bound": "fp32-valu", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4),

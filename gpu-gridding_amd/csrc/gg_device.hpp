@@ -21,7 +21,8 @@ constexpr int kBlock = kWaves * 64;
 
 // ---- Per-visibility record ------------------------------------------------
 // meta  = int4 {grid_u, grid_v, packed, table_base}
-//   packed bits 0-3 flags, 4-15 support, 16-31 kernel-table id (for ordering)
+//   packed bits 0-4 flags, 5-16 support, 17-31 kernel-table id (for
+//   ordering; shifted right as needed to fit 15 bits)
 //   table_base = element offset of the expanded kernel table in use
 // amp   = T4 {a, b, c, d}: the four FMA coefficients of the complex MAC
 //   re += c_re*a + c_im*b ;  im += c_re*c + c_im*d
@@ -29,10 +30,15 @@ constexpr int kFlagNegV = 1;   // off_v < 0: table rows read mirrored
 constexpr int kFlagNegU = 2;   // off_u < 0: table columns read mirrored
 constexpr int kFlagConj = 4;   // ww > 0: Im(kernel) negated
 constexpr int kFlagSkip = 8;   // footprint leaves the grid
+constexpr int kFlagNonFinite = 16;  // weight*vis not finite: kept out of the
+                                    // tiled path and gridded by k_grid_nonfinite
 constexpr int kMaxSupport = 4095;
+constexpr int kSupportShift = 5;
+constexpr int kTableIdShift = 17;
+constexpr int kTableIdBits = 15;
 
-__device__ __forceinline__ int meta_support(int packed) { return (packed >> 4) & 0xFFF; }
-__device__ __forceinline__ unsigned meta_table_id(int packed) { return (unsigned)packed >> 16; }
+__device__ __forceinline__ int meta_support(int packed) { return (packed >> kSupportShift) & 0xFFF; }
+__device__ __forceinline__ unsigned meta_table_id(int packed) { return (unsigned)packed >> kTableIdShift; }
 
 template <typename T> struct V2;
 template <> struct V2<float> { typedef float2 t; };

@@ -110,11 +110,11 @@ struct SelectArgs
     const unsigned* job_chunk_off;   // [num_jobs + 1]
     int num_planes; const int* support; int oversample; int half_ov;
     const double* norm_table; const unsigned* plane_off;
-    int table_id_shift;              // table id >> shift fits 16 bits
+    int table_id_shift;              // table id >> shift fits kTableIdBits
     // outputs
     int4* meta; typename V4<TA>::t* amp; unsigned* num_visits;
     double* part_norm; unsigned long long* part_skipped;
-    unsigned long long* part_taps; int4* part_bbox;
+    unsigned long long* part_taps; unsigned long long* part_nonfinite; int4* part_bbox;
 };
 
 __device__ __forceinline__ int find_job(const unsigned* off, int n, unsigned x)
@@ -180,7 +180,7 @@ void k_select(const SelectArgs<TI, TA> A, unsigned num_chunks)
         const unsigned long long first = J.vis_begin +
                 (unsigned long long)(c - A.job_chunk_off[j]) * kSelChunk;
         double nsum = 0.0;
-        unsigned long long nskip = 0, ntaps = 0;
+        unsigned long long nskip = 0, ntaps = 0, nnonfin = 0;
         int bx0 = INT_MAX, by0 = INT_MAX, bx1 = INT_MAX, by1 = INT_MAX; // max as -min
         for (int q = 0; q < kSelPerThread; ++q)
         {
@@ -203,12 +203,8 @@ void k_select(const SelectArgs<TI, TA> A, unsigned num_chunks)
             const int W = 2 * s.support + 1;
             const unsigned tab = (unsigned)(a * noff + b);
             const unsigned table_id = (unsigned)s.plane * noff * noff + tab;
-            const int flags = (s.off_v < 0 ? kFlagNegV : 0) |
+            int flags = (s.off_v < 0 ? kFlagNegV : 0) |
                     (s.off_u < 0 ? kFlagNegU : 0) | (s.conj ? kFlagConj : 0);
-            m.z = (int)(((table_id >> A.table_id_shift) << 16) |
-                    ((unsigned)s.support << 4) | (unsigned)flags);
-            m.w = (int)(A.plane_off[s.plane] + tab * (unsigned)W * (unsigned)W);
-            A.meta[i] = m;
             const TI w_in = A.weight[i];
             if (A.vis)
             {
@@ -224,13 +220,24 @@ void k_select(const SelectArgs<TI, TA> A, unsigned num_chunks)
                 am.z = v_im;
                 am.w = s.conj ? -v_re : v_re;
                 A.amp[i] = am;
+                if (!isfinite(v_re) || !isfinite(v_im))
+                {
+                    flags |= kFlagNonFinite;
+                    ++nnonfin;
+                }
             }
-            // Tile visits.
+            m.z = (int)(((table_id >> A.table_id_shift) << kTableIdShift) |
+                    ((unsigned)s.support << kSupportShift) | (unsigned)flags);
+            m.w = (int)(A.plane_off[s.plane] + tab * (unsigned)W * (unsigned)W);
+            A.meta[i] = m;
+            // Tile visits (none for non-finite amplitudes: k_grid_nonfinite
+            // grids those after the tiles are done).
             const int tx0 = (s.grid_u - s.support) >> kLogTileW;
             const int tx1 = (s.grid_u + s.support) >> kLogTileW;
             const int ty0 = (s.grid_v - s.support) >> kLogTileH;
             const int ty1 = (s.grid_v + s.support) >> kLogTileH;
-            A.num_visits[i] = (unsigned)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
+            A.num_visits[i] = (flags & kFlagNonFinite) ? 0u
+                    : (unsigned)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
             // Norm term (oskar_grid_wproj.c:210), taps, bounding box.
             nsum += A.norm_table[(s.plane * side + s.off_v + A.half_ov) * side +
                     s.off_u + A.half_ov] * (double)w_in;
@@ -243,6 +250,7 @@ void k_select(const SelectArgs<TI, TA> A, unsigned num_chunks)
         nsum = block_sum(nsum, sh_d);
         nskip = block_sum(nskip, sh_u);
         ntaps = block_sum(ntaps, sh_u);
+        nnonfin = block_sum(nnonfin, sh_u);
         bx0 = block_min(bx0, sh_i);
         by0 = block_min(by0, sh_i);
         bx1 = block_min(bx1, sh_i);
@@ -252,6 +260,7 @@ void k_select(const SelectArgs<TI, TA> A, unsigned num_chunks)
             A.part_norm[c] = nsum;
             A.part_skipped[c] = nskip;
             A.part_taps[c] = ntaps;
+            A.part_nonfinite[c] = nnonfin;
             A.part_bbox[c] = make_int4(bx0, by0, bx1, by1);
         }
     }
@@ -284,8 +293,8 @@ void k_select_out(const TI* __restrict__ uu, const TI* __restrict__ vv,
 }
 
 // Per-job totals over the job's chunks, in chunk order.  job_out receives
-// {skipped, taps} and job_bbox the {x0, y0, x1, y1} cell box (x1 < x0 if
-// nothing is gridded).  *norm += sum; *skipped = count.
+// {skipped, taps, non-finite} and job_bbox the {x0, y0, x1, y1} cell box
+// (x1 < x0 if nothing is gridded).  *norm += sum; *skipped = count.
 template <typename T>
 __global__ __launch_bounds__(kSelThreads)
 void k_job_reduce(const JobDev<T>* __restrict__ jobs,
@@ -293,6 +302,7 @@ void k_job_reduce(const JobDev<T>* __restrict__ jobs,
         const double* __restrict__ part_norm,
         const unsigned long long* __restrict__ part_skipped,
         const unsigned long long* __restrict__ part_taps,
+        const unsigned long long* __restrict__ part_nonfinite,
         const int4* __restrict__ part_bbox,
         unsigned long long* __restrict__ job_out, int4* __restrict__ job_bbox)
 {
@@ -302,27 +312,30 @@ void k_job_reduce(const JobDev<T>* __restrict__ jobs,
     const int j = blockIdx.x;
     const unsigned c0 = job_chunk_off[j], c1 = job_chunk_off[j + 1];
     double nsum = 0.0;
-    unsigned long long nskip = 0, ntaps = 0;
+    unsigned long long nskip = 0, ntaps = 0, nnf = 0;
     int b0 = INT_MAX, b1 = INT_MAX, b2 = INT_MAX, b3 = INT_MAX;
     for (unsigned c = c0 + threadIdx.x; c < c1; c += kSelThreads)
     {
         nsum += part_norm[c];
         nskip += part_skipped[c];
         ntaps += part_taps[c];
+        nnf += part_nonfinite[c];
         const int4 b = part_bbox[c];
         b0 = min(b0, b.x); b1 = min(b1, b.y); b2 = min(b2, b.z); b3 = min(b3, b.w);
     }
     nsum = block_sum(nsum, sh_d);
     nskip = block_sum(nskip, sh_u);
     ntaps = block_sum(ntaps, sh_u);
+    nnf = block_sum(nnf, sh_u);
     b0 = block_min(b0, sh_i); b1 = block_min(b1, sh_i);
     b2 = block_min(b2, sh_i); b3 = block_min(b3, sh_i);
     if (threadIdx.x == 0)
     {
         *jobs[j].norm += nsum;
         *jobs[j].skipped = nskip;
-        job_out[2 * j] = nskip;
-        job_out[2 * j + 1] = ntaps;
+        job_out[3 * j] = nskip;
+        job_out[3 * j + 1] = ntaps;
+        job_out[3 * j + 2] = nnf;
         job_bbox[j] = make_int4(b0, b1, b2 == INT_MAX ? INT_MIN : -b2,
                 b3 == INT_MAX ? INT_MIN : -b3);
     }
@@ -625,6 +638,51 @@ __global__ void k_tile_bounds(const unsigned* __restrict__ keys,
     }
 }
 
+// Runs: maximal stretches of sorted visits with the same tile and the same
+// kernel table.  flag[p] = 1 where a run starts.  When the sort key holds
+// the full table id, key equality decides; otherwise the table bases are
+// compared through the visit records.
+__global__ void k_run_flags(const unsigned* __restrict__ keys,
+        const unsigned* __restrict__ vals, const int4* __restrict__ meta,
+        const unsigned long long* __restrict__ d_n, int exact_key,
+        unsigned* __restrict__ flag)
+{
+    const unsigned long long n = *d_n;
+    for (unsigned long long p = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
+            p < n; p += (unsigned long long)gridDim.x * blockDim.x)
+    {
+        unsigned f = p == 0 || keys[p] != keys[p - 1];
+        if (!f && !exact_key) f = meta[vals[p]].w != meta[vals[p - 1]].w;
+        flag[p] = f;
+    }
+}
+
+// run_start[r] = first visit of run r (excl = exclusive scan of flag);
+// run_start[R] = n.
+__global__ void k_run_starts(const unsigned* __restrict__ flag,
+        const unsigned* __restrict__ excl, const unsigned long long* __restrict__ d_n,
+        const unsigned long long* __restrict__ d_runs, unsigned* __restrict__ run_start)
+{
+    const unsigned long long n = *d_n;
+    for (unsigned long long p = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
+            p < n; p += (unsigned long long)gridDim.x * blockDim.x)
+    {
+        if (flag[p]) run_start[excl[p]] = (unsigned)p;
+        if (p == 0) run_start[*d_runs] = (unsigned)n;
+    }
+}
+
+// run_end[p] = one past the last visit of p's run (written over flag).
+__global__ void k_run_end(unsigned* __restrict__ flag_then_end,
+        const unsigned* __restrict__ excl, const unsigned long long* __restrict__ d_n,
+        const unsigned* __restrict__ run_start)
+{
+    const unsigned long long n = *d_n;
+    for (unsigned long long p = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
+            p < n; p += (unsigned long long)gridDim.x * blockDim.x)
+        flag_then_end[p] = run_start[excl[p] + flag_then_end[p]];
+}
+
 // Work items per tile: ceil(visits / visits_per_item); tiles with two or more
 // items also take that many partial-tile slots.
 __global__ void k_items(const unsigned* __restrict__ tile_start,
@@ -670,6 +728,8 @@ struct GridArgs
     const JobDev<T>* jobs; const unsigned* job_tile_base; int num_jobs;
     unsigned visits_per_item;
     void* scratch;                 // partial tiles of multi-item tiles
+    const unsigned* run_end;       // per sorted visit: end of its run (staged kernel)
+    unsigned min_stage_run;        // shortest run staged through LDS
 };
 
 // TwoSum (Knuth): s + e == a + b exactly.
@@ -793,55 +853,34 @@ void k_grid_tiles(const GridArgs<T> A, unsigned grid_padded)
             const int W = 2 * s + 1;
             const int k = x0 + lane - mt.x;
             const bool neg_u = mt.z & kFlagNegU, neg_v = mt.z & kFlagNegV;
-            const T2* tab = A.tables + (unsigned)mt.w;
             if (k >= -s && k <= s)
             {
                 const unsigned kcol = (unsigned)(neg_u ? s - k : s + k);
                 const unsigned kbyte = kcol * (unsigned)sizeof(T2);
-                if (all_finite<T>(a))
+                // (amplitudes are finite: non-finite ones never reach a tile)
+                #pragma unroll
+                for (int g = 0; g < kRowsPerWave; g += 4)
                 {
+                    const int jg = j0 + g;
+                    if (jg > s || jg + 3 < -s) continue;
+                    T2 c[4];
                     #pragma unroll
-                    for (int g = 0; g < kRowsPerWave; g += 4)
+                    for (int r = 0; r < 4; ++r)
                     {
-                        const int jg = j0 + g;
-                        if (jg > s || jg + 3 < -s) continue;
-                        T2 c[4];
-                        #pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                        {
-                            const int jr = jg + r;
-                            const unsigned soff = (jr >= -s && jr <= s)
-                                    ? ((unsigned)mt.w + (unsigned)((neg_v ? s - jr : s + jr) * W)) *
-                                      (unsigned)sizeof(T2)
-                                    : A.zero_row_byte;
-                            c[r] = load_tap(rs, kbyte, soff, (T2*)nullptr);
-                        }
-                        #pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                        {
-                            acc_re[g + r] = fma(c[r].x, a.x, acc_re[g + r]);
-                            acc_re[g + r] = fma(c[r].y, a.y, acc_re[g + r]);
-                            acc_im[g + r] = fma(c[r].x, a.z, acc_im[g + r]);
-                            acc_im[g + r] = fma(c[r].y, a.w, acc_im[g + r]);
-                        }
+                        const int jr = jg + r;
+                        const unsigned soff = (jr >= -s && jr <= s)
+                                ? ((unsigned)mt.w + (unsigned)((neg_v ? s - jr : s + jr) * W)) *
+                                  (unsigned)sizeof(T2)
+                                : A.zero_row_byte;
+                        c[r] = load_tap(rs, kbyte, soff, (T2*)nullptr);
                     }
-                }
-                else
-                {
-                    // Non-finite amplitude: touch the footprint's own rows
-                    // only, as the serial code does (0 * Inf is NaN).
                     #pragma unroll
-                    for (int r = 0; r < kRowsPerWave; ++r)
+                    for (int r = 0; r < 4; ++r)
                     {
-                        const int jr = j0 + r;
-                        if (jr >= -s && jr <= s)
-                        {
-                            const T2 c = tab[(neg_v ? s - jr : s + jr) * W + kcol];
-                            acc_re[r] = fma(c.x, a.x, acc_re[r]);
-                            acc_re[r] = fma(c.y, a.y, acc_re[r]);
-                            acc_im[r] = fma(c.x, a.z, acc_im[r]);
-                            acc_im[r] = fma(c.y, a.w, acc_im[r]);
-                        }
+                        acc_re[g + r] = fma(c[r].x, a.x, acc_re[g + r]);
+                        acc_re[g + r] = fma(c[r].y, a.y, acc_re[g + r]);
+                        acc_im[g + r] = fma(c[r].x, a.z, acc_im[g + r]);
+                        acc_im[g + r] = fma(c[r].y, a.w, acc_im[g + r]);
                     }
                 }
             }
@@ -963,6 +1002,53 @@ void k_merge(const unsigned* __restrict__ item_off, const unsigned* __restrict__
                 v.x = (T)((double)v.x + re);
                 v.y = (T)((double)v.y + im);
                 *g = v;
+            }
+        }
+    }
+}
+
+// Visibilities whose weighted amplitude is not finite: one wave per
+// visibility adds its footprint with float atomics, after the tiled path
+// has finished.  The cells it touches end up NaN or +-Inf whatever the
+// order of the additions, exactly as the serial code leaves them; keeping
+// these few visibilities out of the tiled kernels lets those read zero
+// rows for out-of-footprint taps (0 * Inf would be NaN).
+template <typename T>
+__global__ __launch_bounds__(256)
+void k_grid_nonfinite(const typename V2<T>::t* __restrict__ tables,
+        const int4* __restrict__ meta, const typename V4<T>::t* __restrict__ amp,
+        const JobDev<T>* __restrict__ jobs, const unsigned long long* __restrict__ job_vis_off,
+        int num_jobs, unsigned long long n)
+{
+    typedef typename V2<T>::t T2;
+    const int lane = lane_id();
+    const unsigned long long nw = (unsigned long long)gridDim.x * (blockDim.x >> 6);
+    const unsigned long long w0 = (unsigned long long)blockIdx.x * (blockDim.x >> 6) +
+            (unsigned)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    for (unsigned long long i = w0; i < n; i += nw)
+    {
+        const int4 mt = ld_const(meta, i);
+        if ((mt.z & (kFlagNonFinite | kFlagSkip)) != kFlagNonFinite) continue;
+        int lo = 0, hi = num_jobs - 1;
+        while (lo < hi)
+        {
+            const int mid = (lo + hi + 1) >> 1;
+            if (ld_const(job_vis_off, mid) <= i) lo = mid; else hi = mid - 1;
+        }
+        const JobDev<T> J = ld_const(jobs, lo);
+        const typename V4<T>::t a = ld_const(amp, i);
+        const int s = meta_support(mt.z);
+        const int W = 2 * s + 1;
+        const bool neg_u = mt.z & kFlagNegU, neg_v = mt.z & kFlagNegV;
+        for (int jr = -s; jr <= s; ++jr)
+        {
+            const T2* krow = tables + (unsigned)mt.w + (neg_v ? s - jr : s + jr) * W;
+            T* grow = reinterpret_cast<T*>(J.grid + (size_t)(mt.y + jr) * J.grid_size + mt.x);
+            for (int k = -s + lane; k <= s; k += 64)
+            {
+                const T2 c = krow[neg_u ? s - k : s + k];
+                atomicAdd(grow + 2 * k, fma(c.y, a.y, c.x * a.x));
+                atomicAdd(grow + 2 * k + 1, fma(c.y, a.w, c.x * a.z));
             }
         }
     }

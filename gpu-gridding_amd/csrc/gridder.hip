@@ -5,6 +5,7 @@
 // of the call's visibilities, on the context's stream, with two small
 // device-to-host reads (visit total, work-item total) to size buffers.
 #include "gg_kernels.hpp"
+#include "gg_grid_staged.hpp"
 #include "mi355_gridder.h"
 
 #include <algorithm>
@@ -131,6 +132,8 @@ struct mi355_ctx
     bool compensated = false;
     bool timing = false;
     bool cache_kernels = false;
+    int grid_kernel = 0;            // 0 staged, 1 direct
+    unsigned min_stage_run = 2;
 
     // prepared kernel tables
     const void* cache_conv = nullptr;
@@ -143,10 +146,10 @@ struct mi355_ctx
 
     DevBuf tables, norm_table, support_d, plane_off_d;
     DevBuf meta, amp, num_visits, visit_off;
-    DevBuf part_norm, part_skipped, part_taps, part_bbox;
+    DevBuf part_norm, part_skipped, part_taps, part_nonfinite, part_bbox;
     DevBuf scan_partial, scalars;
     DevBuf keys[2], vals[2], hist, hist_off;
-    DevBuf tile_start, tile_end, n_items, n_slots, item_off, slot_off, item_tile;
+    DevBuf tile_start, tile_end, n_items, n_slots, item_off, slot_off, item_tile, run_start;
     DevBuf scratch;
     DevBuf jobs_d, job_chunk_off_d, job_vis_off_d, job_tile_base_d, job_out_d, job_bbox_d;
     // drop-in staging
@@ -330,7 +333,7 @@ int scan_u32(mi355_ctx* C, const unsigned* in, unsigned* out, DevCount cnt,
 }
 
 // Device scalars (u64 slots in C->scalars).
-enum Scalar { SC_VISITS, SC_OVERFLOW, SC_ITEMS, SC_SLOTS, SC_SCRATCH0, SC_COUNT = 8 };
+enum Scalar { SC_VISITS, SC_OVERFLOW, SC_ITEMS, SC_SLOTS, SC_SCRATCH0, SC_RUNS = 6, SC_COUNT = 8 };
 
 // ---------------------------------------------------------------------------
 // One pass of the pipeline over num_jobs jobs.
@@ -346,6 +349,23 @@ struct JobHost
     double* norm;
     unsigned long long* skipped;
 };
+
+// Visibilities with non-finite weighted amplitudes (counted by selection),
+// gridded after the tiled path on the same stream.
+template <typename T>
+int grid_nonfinite(mi355_ctx* C, unsigned long long count, int num_jobs,
+        unsigned long long n_total)
+{
+    if (!count) return MI355_OK;
+    typedef typename V2<T>::t T2;
+    typedef typename V4<T>::t T4;
+    hipLaunchKernelGGL(k_grid_nonfinite<T>, dim3(grid_blocks(n_total, 4, 65535)), dim3(256), 0,
+            C->stream, C->tables.as<T2>(), C->meta.as<int4>(), C->amp.as<T4>(),
+            C->jobs_d.as<JobDev<T>>(), C->job_vis_off_d.as<unsigned long long>(), num_jobs,
+            n_total);
+    HIP_TRY(hipGetLastError());
+    return MI355_OK;
+}
 
 // Called once the gridded bounding box of each job is known (before the
 // grid is touched); the host drop-in uploads the caller's grid window here.
@@ -421,7 +441,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     if (tile_bits > 32) return fail(MI355_ERR_TOO_LARGE, "tile id exceeds 32 bits");
     const unsigned long long num_tables = (unsigned long long)P * noff * noff;
     const int table_bits_full = bits_for(num_tables);
-    const int table_id_shift = std::max(0, table_bits_full - 16);
+    const int table_id_shift = std::max(0, table_bits_full - kTableIdBits);
     const int key_table_bits = std::min(std::max(0, table_bits_full - table_id_shift), 32 - tile_bits);
     const int key_table_shift = std::max(0, table_bits_full - table_id_shift - key_table_bits);
 
@@ -429,7 +449,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     TRY(ensure(C->job_chunk_off_d, (num_jobs + 1) * sizeof(unsigned)));
     TRY(ensure(C->job_tile_base_d, (num_jobs + 1) * sizeof(unsigned)));
     TRY(ensure(C->job_vis_off_d, (num_jobs + 1) * sizeof(unsigned long long)));
-    TRY(ensure(C->job_out_d, 2 * num_jobs * sizeof(unsigned long long)));
+    TRY(ensure(C->job_out_d, 3 * num_jobs * sizeof(unsigned long long)));
     TRY(ensure(C->job_bbox_d, num_jobs * sizeof(int4)));
     HIP_TRY(hipMemcpyAsync(C->jobs_d.p, C->h_jobs.data(), C->h_jobs.size(), hipMemcpyHostToDevice, C->stream));
     HIP_TRY(hipMemcpyAsync(C->job_chunk_off_d.p, C->h_job_chunk_off.data(), (num_jobs + 1) * sizeof(unsigned), hipMemcpyHostToDevice, C->stream));
@@ -444,6 +464,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     TRY(ensure(C->part_norm, chunks * sizeof(double)));
     TRY(ensure(C->part_skipped, chunks * sizeof(unsigned long long)));
     TRY(ensure(C->part_taps, chunks * sizeof(unsigned long long)));
+    TRY(ensure(C->part_nonfinite, chunks * sizeof(unsigned long long)));
     TRY(ensure(C->part_bbox, chunks * sizeof(int4)));
     TRY(ensure(C->scalars, SC_COUNT * sizeof(unsigned long long)));
     unsigned long long* sc = C->scalars.as<unsigned long long>();
@@ -464,6 +485,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     sa.part_norm = C->part_norm.as<double>();
     sa.part_skipped = C->part_skipped.as<unsigned long long>();
     sa.part_taps = C->part_taps.as<unsigned long long>();
+    sa.part_nonfinite = C->part_nonfinite.as<unsigned long long>();
     sa.part_bbox = C->part_bbox.as<int4>();
     if (chunks)
     {
@@ -474,8 +496,9 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     hipLaunchKernelGGL(k_job_reduce<T>, dim3(num_jobs), dim3(kSelThreads), 0, C->stream,
             C->jobs_d.as<JobDev<T>>(), C->job_chunk_off_d.as<unsigned>(),
             C->part_norm.as<double>(), C->part_skipped.as<unsigned long long>(),
-            C->part_taps.as<unsigned long long>(), C->part_bbox.as<int4>(),
-            C->job_out_d.as<unsigned long long>(), C->job_bbox_d.as<int4>());
+            C->part_taps.as<unsigned long long>(), C->part_nonfinite.as<unsigned long long>(),
+            C->part_bbox.as<int4>(), C->job_out_d.as<unsigned long long>(),
+            C->job_bbox_d.as<int4>());
     HIP_TRY(hipGetLastError());
     record(C, EV_SELECT);
 
@@ -486,26 +509,28 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
                 n_total, sc + SC_VISITS, reinterpret_cast<unsigned*>(sc + SC_OVERFLOW), 1));
     }
     unsigned long long host_sc[SC_COUNT];
-    C->h_job_out.resize(2 * num_jobs);
+    C->h_job_out.resize(3 * num_jobs);
     C->h_job_bbox.resize(num_jobs);
     HIP_TRY(hipMemcpyAsync(host_sc, sc, sizeof(host_sc), hipMemcpyDeviceToHost, C->stream));
-    HIP_TRY(hipMemcpyAsync(C->h_job_out.data(), C->job_out_d.p, 2 * num_jobs * sizeof(unsigned long long), hipMemcpyDeviceToHost, C->stream));
+    HIP_TRY(hipMemcpyAsync(C->h_job_out.data(), C->job_out_d.p, 3 * num_jobs * sizeof(unsigned long long), hipMemcpyDeviceToHost, C->stream));
     HIP_TRY(hipMemcpyAsync(C->h_job_bbox.data(), C->job_bbox_d.p, num_jobs * sizeof(int4), hipMemcpyDeviceToHost, C->stream));
     HIP_TRY(hipStreamSynchronize(C->stream));
     if (host_sc[SC_OVERFLOW]) return fail(MI355_ERR_TOO_LARGE, "more than 2^32-1 tile visits in one call");
     const unsigned long long T_vis = host_sc[SC_VISITS];
+    unsigned long long num_nonfinite = 0;
     C->stats.num_points = n_total;
     for (int j = 0; j < num_jobs; ++j)
     {
-        C->stats.num_skipped += C->h_job_out[2 * j];
-        C->stats.num_taps += C->h_job_out[2 * j + 1];
+        C->stats.num_skipped += C->h_job_out[3 * j];
+        C->stats.num_taps += C->h_job_out[3 * j + 1];
+        num_nonfinite += C->h_job_out[3 * j + 2];
     }
     C->stats.num_visits = T_vis;
     if (before_grid) TRY((*before_grid)(C, C->h_job_bbox.data()));
     if (T_vis == 0)
     {
         record(C, EV_BUCKET); record(C, EV_GRID); record(C, EV_MERGE);
-        return MI355_OK;
+        return grid_nonfinite<T>(C, num_nonfinite, num_jobs, n_total);
     }
 
     // Emit (tile | table) keys and sort them.
@@ -540,6 +565,30 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     }
     const unsigned* skeys = C->keys[cur].as<unsigned>();
     const unsigned* svals = C->vals[cur].as<unsigned>();
+
+    // Runs of visits sharing a tile and a kernel table (staged kernel only):
+    // flags -> exclusive scan -> run starts -> per-visit run end, in the
+    // sort's free ping-pong buffers.
+    unsigned* run_end = nullptr;
+    if (C->grid_kernel == 0)
+    {
+        unsigned* flag = C->keys[cur ^ 1].as<unsigned>();
+        unsigned* excl = C->vals[cur ^ 1].as<unsigned>();
+        const int exact_key = table_id_shift == 0 && key_table_shift == 0;
+        const unsigned vb = grid_blocks(T_vis, 256, 65535);
+        hipLaunchKernelGGL(k_run_flags, dim3(vb), dim3(256), 0, C->stream, skeys, svals,
+                C->meta.as<int4>(), sc + SC_VISITS, exact_key, flag);
+        DevCount rc{sc + SC_VISITS, 1, 1, 0};
+        TRY(scan_u32(C, flag, excl, rc, T_vis, sc + SC_RUNS,
+                reinterpret_cast<unsigned*>(sc + SC_OVERFLOW), 0));
+        TRY(ensure(C->run_start, (T_vis + 1) * sizeof(unsigned)));
+        hipLaunchKernelGGL(k_run_starts, dim3(vb), dim3(256), 0, C->stream, flag, excl,
+                sc + SC_VISITS, sc + SC_RUNS, C->run_start.as<unsigned>());
+        hipLaunchKernelGGL(k_run_end, dim3(vb), dim3(256), 0, C->stream, flag, excl,
+                sc + SC_VISITS, C->run_start.as<unsigned>());
+        HIP_TRY(hipGetLastError());
+        run_end = flag;
+    }
 
     // Tiles -> work items.
     TRY(ensure(C->tile_start, num_tiles * sizeof(unsigned)));
@@ -596,8 +645,28 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     ga.num_jobs = num_jobs;
     ga.visits_per_item = C->visits_per_item;
     ga.scratch = C->scratch.p;
+    ga.run_end = run_end;
+    ga.min_stage_run = C->min_stage_run;
     const unsigned padded = (unsigned)((num_items + 7) & ~7ull);
-    if (num_items)
+    if (num_items && C->grid_kernel == 0)
+    {
+        // One workgroup per work item, in item order (consecutive items of a
+        // heavy tile land on different XCDs).
+        if constexpr (std::is_same<T, float>::value)
+        {
+            if (comp)
+                hipLaunchKernelGGL((k_grid_staged<T, true>), dim3((unsigned)num_items),
+                        dim3(kStageBlock), 0, C->stream, ga);
+            else
+                hipLaunchKernelGGL((k_grid_staged<T, false>), dim3((unsigned)num_items),
+                        dim3(kStageBlock), 0, C->stream, ga);
+        }
+        else
+            hipLaunchKernelGGL((k_grid_staged<T, false>), dim3((unsigned)num_items),
+                    dim3(kStageBlock), 0, C->stream, ga);
+        HIP_TRY(hipGetLastError());
+    }
+    else if (num_items)
     {
         if (comp)
         {
@@ -626,7 +695,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
         HIP_TRY(hipGetLastError());
     }
     record(C, EV_MERGE);
-    return MI355_OK;
+    return grid_nonfinite<T>(C, num_nonfinite, num_jobs, n_total);
 }
 
 void collect_timing(mi355_ctx* C)
@@ -837,6 +906,7 @@ int degrid_dev(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     TRY(ensure(C->part_norm, chunks * sizeof(double)));
     TRY(ensure(C->part_skipped, chunks * sizeof(unsigned long long)));
     TRY(ensure(C->part_taps, chunks * sizeof(unsigned long long)));
+    TRY(ensure(C->part_nonfinite, chunks * sizeof(unsigned long long)));
     TRY(ensure(C->part_bbox, chunks * sizeof(int4)));
     SelectArgs<T, T> sa;
     sa.uu = uu; sa.vv = vv; sa.ww = ww; sa.weight = weight;
@@ -847,12 +917,13 @@ int degrid_dev(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     sa.oversample = ov; sa.half_ov = half;
     sa.norm_table = C->norm_table.as<double>();
     sa.plane_off = C->plane_off_d.as<unsigned>();
-    sa.table_id_shift = std::max(0, bits_for((unsigned long long)P * noff * noff) - 16);
+    sa.table_id_shift = std::max(0, bits_for((unsigned long long)P * noff * noff) - kTableIdBits);
     sa.meta = C->meta.as<int4>(); sa.amp = C->amp.as<T4>();
     sa.num_visits = C->num_visits.as<unsigned>();
     sa.part_norm = C->part_norm.as<double>();
     sa.part_skipped = C->part_skipped.as<unsigned long long>();
     sa.part_taps = C->part_taps.as<unsigned long long>();
+    sa.part_nonfinite = C->part_nonfinite.as<unsigned long long>();
     sa.part_bbox = C->part_bbox.as<int4>();
     hipLaunchKernelGGL((k_select<T, T>), dim3(grid_blocks(chunks, 1, 65535)), dim3(kSelThreads), 0,
             C->stream, sa, (unsigned)chunks);
@@ -932,9 +1003,11 @@ MI355_EXPORT int mi355_ctx_destroy(mi355_ctx* C)
     (void)hipStreamSynchronize(C->stream);
     DevBuf* all[] = {&C->tables, &C->norm_table, &C->support_d, &C->plane_off_d, &C->meta,
         &C->amp, &C->num_visits, &C->visit_off, &C->part_norm, &C->part_skipped,
-        &C->part_taps, &C->part_bbox, &C->scan_partial, &C->scalars, &C->keys[0], &C->keys[1],
+        &C->part_taps, &C->part_nonfinite, &C->part_bbox, &C->scan_partial, &C->scalars,
+        &C->keys[0], &C->keys[1],
         &C->vals[0], &C->vals[1], &C->hist, &C->hist_off, &C->tile_start, &C->tile_end,
-        &C->n_items, &C->n_slots, &C->item_off, &C->slot_off, &C->item_tile, &C->scratch,
+        &C->n_items, &C->n_slots, &C->item_off, &C->slot_off, &C->item_tile, &C->run_start,
+        &C->scratch,
         &C->jobs_d, &C->job_chunk_off_d, &C->job_vis_off_d, &C->job_tile_base_d,
         &C->job_out_d, &C->job_bbox_d, &C->in_conv, &C->in_uu, &C->in_vv, &C->in_ww,
         &C->in_vis, &C->in_weight, &C->in_grid, &C->in_norm, &C->in_skipped};
@@ -963,6 +1036,14 @@ MI355_EXPORT int mi355_ctx_set_option(mi355_ctx* C, int key, long long value)
     case MI355_OPT_COMPENSATED: C->compensated = value != 0; break;
     case MI355_OPT_PHASE_TIMING: C->timing = value != 0; break;
     case MI355_OPT_CACHE_KERNELS: C->cache_kernels = value != 0; C->tables_valid = false; break;
+    case MI355_OPT_GRID_KERNEL:
+        if (value < 0 || value > 1) return fail(MI355_ERR_INVALID_ARG, "grid kernel %lld", value);
+        C->grid_kernel = (int)value;
+        break;
+    case MI355_OPT_MIN_STAGE_RUN:
+        if (value < 1) return fail(MI355_ERR_INVALID_ARG, "min stage run %lld", value);
+        C->min_stage_run = (unsigned)std::min<long long>(value, 0xFFFFFFFFll);
+        break;
     default: return fail(MI355_ERR_INVALID_ARG, "unknown option %d", key);
     }
     return ok();

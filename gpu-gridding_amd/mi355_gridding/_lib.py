@@ -25,6 +25,8 @@ OPT_VISITS_PER_ITEM = 1
 OPT_COMPENSATED = 2
 OPT_PHASE_TIMING = 3
 OPT_CACHE_KERNELS = 4
+OPT_GRID_KERNEL = 5
+OPT_MIN_STAGE_RUN = 6
 
 
 class GridderError(RuntimeError):

@@ -59,6 +59,31 @@ def same_bits(a, b):
     return a.shape == b.shape and a.dtype == b.dtype and a.tobytes() == b.tobytes()
 
 
+def same_scalar(a, b):
+    """Equal, or both NaN."""
+    a, b = float(a), float(b)
+    return a == b or (a != a and b != b)
+
+
+def close_scalar(a, b, rel):
+    a, b = float(a), float(b)
+    if a != a or b != b:
+        return a != a and b != b
+    return abs(a - b) <= rel * abs(b)
+
+
+def nonfinite_split(test, truth):
+    """For grids with NaN / Inf cells: True when the NaN cells and the +-Inf
+    cells coincide; returns (ok, finite test values, finite truth values)."""
+    a = np.asarray(test, np.float64)
+    b = np.asarray(truth, np.float64)
+    ok = np.array_equal(np.isnan(a), np.isnan(b)) and \
+        np.array_equal(np.isposinf(a), np.isposinf(b)) and \
+        np.array_equal(np.isneginf(a), np.isneginf(b))
+    fin = np.isfinite(b)
+    return ok, a[fin], b[fin]
+
+
 @pytest.fixture(scope="session")
 def gpu():
     """torch + the gridder library on cuda:0; fails (does not skip) when the

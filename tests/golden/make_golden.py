@@ -84,6 +84,18 @@ def special_case():
     return c
 
 
+def nonfinite_case():
+    """Random case with a few non-finite amplitudes and weights: the cells
+    they touch become NaN / +-Inf in the serial code."""
+    c = random_case(4, 300, 8, 6, 4, 64)
+    c["vis"][5, 0] = np.nan
+    c["vis"][17, 1] = np.inf
+    c["vis"][33, 0] = -np.inf
+    c["weight"][41] = np.inf
+    c["weight"][57] = np.nan
+    return c
+
+
 def synth_case(N=3000, G=256, P=32, smax=12):
     c = synth.make(0, num_points=N, grid_size=G, num_w_planes=P, max_support=smax,
                    seed=12345)
@@ -128,6 +140,7 @@ def main():
     save("special_g64", with_outputs(special_case()))
     save("synth_g256", with_outputs(synth_case()))
     save("accumulate_g64", with_outputs(random_case(3, 300, 8, 6, 4, 64), accumulate=True))
+    save("nonfinite_g64", with_outputs(nonfinite_case()))
 
 
 if __name__ == "__main__":

@@ -15,7 +15,7 @@ against it.
 import numpy as np
 import pytest
 
-from conftest import errors, golden_names, load_golden, same_bits
+from conftest import errors, golden_names, load_golden, same_bits, same_scalar
 from oracle import oracle
 
 
@@ -34,9 +34,9 @@ def test_oracle_matches_reference_fixture(name):
         g, n, s = oracle.grid_f(c)
         gd, nd, sd = oracle.grid_d(c)
     assert same_bits(g, c["grid_f"])
-    assert n == float(c["norm_f"]) and s == int(c["skipped_f"])
+    assert same_scalar(n, c["norm_f"]) and s == int(c["skipped_f"])
     assert same_bits(gd, c["grid_d"])
-    assert nd == float(c["norm_d"]) and sd == int(c["skipped_d"])
+    assert same_scalar(nd, c["norm_d"]) and sd == int(c["skipped_d"])
 
 
 def test_fixtures_cover_selection_edges():
