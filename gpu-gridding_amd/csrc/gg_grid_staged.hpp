@@ -262,7 +262,8 @@ void k_grid_staged(const GridArgs<T> A)
             const unsigned bend = min(b + kStageLanes * kBatch, v1);
             for (unsigned p = b; p < bend;)
             {
-                const unsigned e = min(ld_const(A.run_end, p), bend);
+                // (max: every step advances, whatever run_end holds)
+                const unsigned e = max(min(ld_const(A.run_end, p), bend), p + 1);
                 const int4 m0 = ld_const(A.meta, ld_const(A.visit_vis, p));
                 const int W0 = 2 * meta_support(m0.z) + 1;
                 bool staged = m0.w == staged_base;
