@@ -207,6 +207,7 @@ void k_grid_staged(const GridArgs<T> A)
     static_assert(stage_lds_elems<T>() >= kWaves * kRowsPerWave * 64 * (COMP ? 2 : 1),
                   "combine buffer must fit in the table buffer");
     const unsigned num_items = ld_const(A.d_num_items, 0);
+    const unsigned num_runs = (unsigned)ld_const(A.d_num_runs, 0);
     const int lane = lane_id();
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int strip = wave & (kWaves - 1);
@@ -250,6 +251,24 @@ void k_grid_staged(const GridArgs<T> A)
         int since = 0;
         int staged_base = -1;          // table base currently in LDS (per item)
 
+        // Runs of this item: the run of v0 onwards, their records fetched 64
+        // at a time (lane l holds run rbase + l) and broadcast.
+        unsigned rbase = ld_const(A.run_excl, v0) + ld_const(A.run_flag, v0) - 1u;
+        unsigned rv_start = 0, rv_next = 0;
+        int rv_z = 0, rv_tb = 0;
+        auto load_runs = [&]() {
+            const unsigned r = rbase + lane;
+            if (r < num_runs)
+            {
+                rv_start = A.run_start[r];
+                rv_next = A.run_start[r + 1];
+                rv_z = A.run_z[r];
+                rv_tb = A.run_tb[r];
+            }
+        };
+        load_runs();
+        int rl = 0;
+
         for (unsigned b = v0; b < v1; b += kStageLanes * kBatch)
         {
             // Records of this wave's visits b + q + 2l, l < 64.
@@ -262,19 +281,28 @@ void k_grid_staged(const GridArgs<T> A)
             const unsigned bend = min(b + kStageLanes * kBatch, v1);
             for (unsigned p = b; p < bend;)
             {
-                // (max: every step advances, whatever run_end holds)
-                const unsigned e = max(min(ld_const(A.run_end, p), bend), p + 1);
-                const int4 m0 = ld_const(A.meta, ld_const(A.visit_vis, p));
-                const int W0 = 2 * meta_support(m0.z) + 1;
-                bool staged = m0.w == staged_base;
-                if (!staged && W0 <= S && ld_const(A.run_end, p) - p >= A.min_stage_run)
+                if (rl == 64)
+                {
+                    rbase += 64;
+                    load_runs();
+                    rl = 0;
+                }
+                const unsigned rstart = (unsigned)bcast((int)rv_start, rl);
+                const unsigned rnext = (unsigned)bcast((int)rv_next, rl);
+                const int rtb = bcast(rv_tb, rl);
+                const int W0 = 2 * meta_support(bcast(rv_z, rl)) + 1;
+                // (max: every step advances, whatever the run table holds)
+                const unsigned e = max(min(rnext, bend), p + 1);
+                bool staged = rtb == staged_base;
+                if (!staged && W0 <= S && rnext - rstart >= A.min_stage_run)
                 {
                     __syncthreads();
-                    stage_table<T>(tab, A.tables + (unsigned)m0.w, W0, wave, lane);
+                    stage_table<T>(tab, A.tables + (unsigned)rtb, W0, wave, lane);
                     __syncthreads();
-                    staged_base = m0.w;
+                    staged_base = rtb;
                     staged = true;
                 }
+                if (e >= rnext) ++rl;      // the run ends in this batch
                 // this wave's visits in [p, e): l with p <= b + q + 2l < e
                 const int lbeg = p - b <= (unsigned)q ? 0 : (int)((p - b - q + 1) / kStageLanes);
                 const int lend = e - b <= (unsigned)q ? 0 : (int)((e - b - q + 1) / kStageLanes);

@@ -65,19 +65,23 @@ __global__ void k_expand_tables(const typename V2<TI>::t* __restrict__ conv,
     }
 }
 
-// norm_table[p][off_v + H][off_u + H] = sum over the footprint of Re(tap),
-// accumulated in double in the reference's order (rows j outer, columns k
-// inner, oskar_grid_wproj.c:190-208), so that each visibility's norm term
-// sum * weight (:210) is bit-identical to the serial code's.
+// norm_table[p][off_v + H][off_u + H] = sum over the footprint of Re(tap) in
+// double: what one visibility with these (plane, off_u, off_v) adds to the
+// norm per unit weight (the `sum` of oskar_grid_wproj.c:154, :207, :210).
+// One wave per entry: lanes take columns, rows in order, then a fixed-order
+// wave reduction (deterministic; agrees with the serial order's double sum
+// to a few ulps).
 template <typename T>
-__global__ void k_norm_table(const typename V2<T>::t* __restrict__ conv,
+__global__ __launch_bounds__(256)
+void k_norm_table(const typename V2<T>::t* __restrict__ conv,
         int num_planes, int conv_size_half, int oversample, int half_ov,
         const int* __restrict__ support, double* __restrict__ norm_table)
 {
     const int side = 2 * half_ov + 1;
     const int total = num_planes * side * side;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total;
-            e += gridDim.x * blockDim.x)
+    const int lane = lane_id();
+    const int nw = gridDim.x * (blockDim.x >> 6);
+    for (int e = blockIdx.x * (blockDim.x >> 6) + (int)(threadIdx.x >> 6); e < total; e += nw)
     {
         const int p = e / (side * side);
         const int r = e - p * side * side;
@@ -88,10 +92,11 @@ __global__ void k_norm_table(const typename V2<T>::t* __restrict__ conv,
         for (int j = -s; j <= s; ++j)
         {
             const typename V2<T>::t* row = src + (size_t)abs(ov + oversample * j) * conv_size_half;
-            for (int k = -s; k <= s; ++k)
+            for (int k = -s + lane; k <= s; k += 64)
                 sum += (double)row[abs(ou + oversample * k)].x;
         }
-        norm_table[e] = sum;
+        sum = wave_sum(sum);
+        if (lane == 0) norm_table[e] = sum;
     }
 }
 
@@ -657,30 +662,29 @@ __global__ void k_run_flags(const unsigned* __restrict__ keys,
     }
 }
 
-// run_start[r] = first visit of run r (excl = exclusive scan of flag);
-// run_start[R] = n.
+// Per run r (excl = exclusive scan of flag): run_start[r] = its first visit,
+// run_z[r] / run_tb[r] = packed flags-support-table word and table base of
+// its visits; run_start[R] = n.  The run of visit p is excl[p] + flag[p] - 1.
 __global__ void k_run_starts(const unsigned* __restrict__ flag,
-        const unsigned* __restrict__ excl, const unsigned long long* __restrict__ d_n,
-        const unsigned long long* __restrict__ d_runs, unsigned* __restrict__ run_start)
+        const unsigned* __restrict__ excl, const unsigned* __restrict__ vals,
+        const int4* __restrict__ meta, const unsigned long long* __restrict__ d_n,
+        const unsigned long long* __restrict__ d_runs, unsigned* __restrict__ run_start,
+        int* __restrict__ run_z, int* __restrict__ run_tb)
 {
     const unsigned long long n = *d_n;
     for (unsigned long long p = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
             p < n; p += (unsigned long long)gridDim.x * blockDim.x)
     {
-        if (flag[p]) run_start[excl[p]] = (unsigned)p;
+        if (flag[p])
+        {
+            const unsigned r = excl[p];
+            const int4 m = meta[vals[p]];
+            run_start[r] = (unsigned)p;
+            run_z[r] = m.z;
+            run_tb[r] = m.w;
+        }
         if (p == 0) run_start[*d_runs] = (unsigned)n;
     }
-}
-
-// run_end[p] = one past the last visit of p's run (written over flag).
-__global__ void k_run_end(unsigned* __restrict__ flag_then_end,
-        const unsigned* __restrict__ excl, const unsigned long long* __restrict__ d_n,
-        const unsigned* __restrict__ run_start)
-{
-    const unsigned long long n = *d_n;
-    for (unsigned long long p = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
-            p < n; p += (unsigned long long)gridDim.x * blockDim.x)
-        flag_then_end[p] = run_start[excl[p] + flag_then_end[p]];
 }
 
 // Work items per tile: ceil(visits / visits_per_item); tiles with two or more
@@ -728,7 +732,10 @@ struct GridArgs
     const JobDev<T>* jobs; const unsigned* job_tile_base; int num_jobs;
     unsigned visits_per_item;
     void* scratch;                 // partial tiles of multi-item tiles
-    const unsigned* run_end;       // per sorted visit: end of its run (staged kernel)
+    // runs (staged kernel): see k_run_starts
+    const unsigned* run_flag; const unsigned* run_excl;
+    const unsigned* run_start; const int* run_z; const int* run_tb;
+    const unsigned long long* d_num_runs;
     unsigned min_stage_run;        // shortest run staged through LDS
 };
 

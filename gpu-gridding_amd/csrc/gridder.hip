@@ -149,7 +149,8 @@ struct mi355_ctx
     DevBuf part_norm, part_skipped, part_taps, part_nonfinite, part_bbox;
     DevBuf scan_partial, scalars;
     DevBuf keys[2], vals[2], hist, hist_off;
-    DevBuf tile_start, tile_end, n_items, n_slots, item_off, slot_off, item_tile, run_start;
+    DevBuf tile_start, tile_end, n_items, n_slots, item_off, slot_off, item_tile;
+    DevBuf run_start, run_z, run_tb;
     DevBuf scratch;
     DevBuf jobs_d, job_chunk_off_d, job_vis_off_d, job_tile_base_d, job_out_d, job_bbox_d;
     // drop-in staging
@@ -300,7 +301,7 @@ int prepare_tables(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
                 C->plane_off_d.as<unsigned>(), C->tables.as<T2>());
         HIP_TRY(hipGetLastError());
         const unsigned entries = (unsigned)(P * (2 * half + 1) * (2 * half + 1));
-        hipLaunchKernelGGL(k_norm_table<TI>, dim3(grid_blocks(entries, 64, 65535)), dim3(64), 0,
+        hipLaunchKernelGGL(k_norm_table<TI>, dim3(grid_blocks(entries, 4, 65535)), dim3(256), 0,
                 C->stream, conv_dev, (int)P, csh, ov, half, C->support_d.as<int>(),
                 C->norm_table.as<double>());
         HIP_TRY(hipGetLastError());
@@ -567,27 +568,26 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     const unsigned* svals = C->vals[cur].as<unsigned>();
 
     // Runs of visits sharing a tile and a kernel table (staged kernel only):
-    // flags -> exclusive scan -> run starts -> per-visit run end, in the
-    // sort's free ping-pong buffers.
-    unsigned* run_end = nullptr;
+    // flags -> exclusive scan -> per-run start, packed word and table base.
+    // Flags and scan live in the sort's free ping-pong buffers.
+    unsigned* run_flag = C->keys[cur ^ 1].as<unsigned>();
+    unsigned* run_excl = C->vals[cur ^ 1].as<unsigned>();
     if (C->grid_kernel == 0)
     {
-        unsigned* flag = C->keys[cur ^ 1].as<unsigned>();
-        unsigned* excl = C->vals[cur ^ 1].as<unsigned>();
         const int exact_key = table_id_shift == 0 && key_table_shift == 0;
         const unsigned vb = grid_blocks(T_vis, 256, 65535);
         hipLaunchKernelGGL(k_run_flags, dim3(vb), dim3(256), 0, C->stream, skeys, svals,
-                C->meta.as<int4>(), sc + SC_VISITS, exact_key, flag);
+                C->meta.as<int4>(), sc + SC_VISITS, exact_key, run_flag);
         DevCount rc{sc + SC_VISITS, 1, 1, 0};
-        TRY(scan_u32(C, flag, excl, rc, T_vis, sc + SC_RUNS,
+        TRY(scan_u32(C, run_flag, run_excl, rc, T_vis, sc + SC_RUNS,
                 reinterpret_cast<unsigned*>(sc + SC_OVERFLOW), 0));
         TRY(ensure(C->run_start, (T_vis + 1) * sizeof(unsigned)));
-        hipLaunchKernelGGL(k_run_starts, dim3(vb), dim3(256), 0, C->stream, flag, excl,
-                sc + SC_VISITS, sc + SC_RUNS, C->run_start.as<unsigned>());
-        hipLaunchKernelGGL(k_run_end, dim3(vb), dim3(256), 0, C->stream, flag, excl,
-                sc + SC_VISITS, C->run_start.as<unsigned>());
+        TRY(ensure(C->run_z, (T_vis + 1) * sizeof(int)));
+        TRY(ensure(C->run_tb, (T_vis + 1) * sizeof(int)));
+        hipLaunchKernelGGL(k_run_starts, dim3(vb), dim3(256), 0, C->stream, run_flag, run_excl,
+                svals, C->meta.as<int4>(), sc + SC_VISITS, sc + SC_RUNS,
+                C->run_start.as<unsigned>(), C->run_z.as<int>(), C->run_tb.as<int>());
         HIP_TRY(hipGetLastError());
-        run_end = flag;
     }
 
     // Tiles -> work items.
@@ -645,7 +645,12 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     ga.num_jobs = num_jobs;
     ga.visits_per_item = C->visits_per_item;
     ga.scratch = C->scratch.p;
-    ga.run_end = run_end;
+    ga.run_flag = run_flag;
+    ga.run_excl = run_excl;
+    ga.run_start = C->run_start.as<unsigned>();
+    ga.run_z = C->run_z.as<int>();
+    ga.run_tb = C->run_tb.as<int>();
+    ga.d_num_runs = sc + SC_RUNS;
     ga.min_stage_run = C->min_stage_run;
     const unsigned padded = (unsigned)((num_items + 7) & ~7ull);
     if (num_items && C->grid_kernel == 0)
@@ -1007,7 +1012,7 @@ MI355_EXPORT int mi355_ctx_destroy(mi355_ctx* C)
         &C->keys[0], &C->keys[1],
         &C->vals[0], &C->vals[1], &C->hist, &C->hist_off, &C->tile_start, &C->tile_end,
         &C->n_items, &C->n_slots, &C->item_off, &C->slot_off, &C->item_tile, &C->run_start,
-        &C->scratch,
+        &C->run_z, &C->run_tb, &C->scratch,
         &C->jobs_d, &C->job_chunk_off_d, &C->job_vis_off_d, &C->job_tile_base_d,
         &C->job_out_d, &C->job_bbox_d, &C->in_conv, &C->in_uu, &C->in_vv, &C->in_ww,
         &C->in_vis, &C->in_weight, &C->in_grid, &C->in_norm, &C->in_skipped};
