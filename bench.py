@@ -73,10 +73,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--kernel", choices=("staged", "direct"), default="staged",
-                    help="tile kernel: kernel tables staged in LDS, or read directly")
+    ap.add_argument("--kernel", choices=("strips", "staged", "direct"), default="strips",
+                    help="gridding kernel: 16-column strips on 128x128 regions, or 64x64 "
+                         "tiles with tables staged in LDS / read directly")
     ap.add_argument("--visits-per-item", type=int, default=2048)
-    ap.add_argument("--min-stage-run", type=int, default=2)
+    ap.add_argument("--min-stage-run", type=int, default=4)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (the gridder has no multi-GPU "
@@ -104,7 +105,7 @@ def main():
     ctx = ops.context()
     ctx.set_option(_lib.OPT_PHASE_TIMING, 1)
     ctx.set_option(_lib.OPT_COMPENSATED, 1 if a.compensated else 0)
-    ctx.set_option(_lib.OPT_GRID_KERNEL, 0 if a.kernel == "staged" else 1)
+    ctx.set_option(_lib.OPT_GRID_KERNEL, {"strips": 0, "staged": 2, "direct": 1}[a.kernel])
     ctx.set_option(_lib.OPT_VISITS_PER_ITEM, a.visits_per_item)
     ctx.set_option(_lib.OPT_MIN_STAGE_RUN, a.min_stage_run)
 
@@ -193,7 +194,9 @@ def main():
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "frac_nonpacked": round(achieved / PEAK_FP32_NONPACKED_TFLOPS, 4),
-                     "kernel": "k_grid_tiles", "kernel_ms": round(grid_ms, 4),
+                     "kernel": {"strips": "k_grid_strips", "staged": "k_grid_staged",
+                                "direct": "k_grid_tiles"}[a.kernel],
+                     "kernel_ms": round(grid_ms, 4),
                      "flops_per_launch": flops, "traffic": None},
         "phases_ms": phase_ms,
         "work": {"num_taps": st["num_taps"], "num_visits": st["num_visits"],

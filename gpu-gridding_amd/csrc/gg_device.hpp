@@ -36,6 +36,15 @@ constexpr int kMaxSupport = 4095;
 constexpr int kSupportShift = 5;
 constexpr int kTableIdShift = 17;
 constexpr int kTableIdBits = 15;
+constexpr int kOrientMask = kFlagNegV | kFlagNegU;   // mirroring of a table read
+// Expanded kernel tables carry this many zero rows above and below each
+// (2s+1)^2 table, so that reads of up to that many rows past a footprint
+// edge return zero taps.
+constexpr int kTablePadRows = 7;
+// The table buffer starts and ends with this many rows of the widest table
+// (zeros), so that reads running up to that many rows past any table stay
+// inside the buffer.
+constexpr int kTableGuardRows = 16;
 
 __device__ __forceinline__ int meta_support(int packed) { return (packed >> kSupportShift) & 0xFFF; }
 __device__ __forceinline__ unsigned meta_table_id(int packed) { return (unsigned)packed >> kTableIdShift; }

@@ -28,7 +28,9 @@ namespace gg {
 // scattered samples.  The table for (p, a, b), a = |off_u|, b = |off_v|,
 // stores that tap at row j + s, column k + s, a dense (2s+1)^2 complex
 // array: a wave reads one contiguous 512-byte row segment.  Negative offsets
-// use the same table mirrored (|-a + ov*k| = |a + ov*(-k)|).
+// use the same table mirrored (|-a + ov*k| = |a + ov*(-k)|).  Each table
+// occupies (W + 2*kTablePadRows) rows of W taps, the pad rows zero; the
+// table's row 0 sits kTablePadRows rows into its block.
 // TI: precision of the caller's kernel stack; TA: precision the gridder
 // accumulates in (the tables are stored in TA; float -> double is exact).
 template <typename TI, typename TA>
@@ -43,7 +45,7 @@ __global__ void k_expand_tables(const typename V2<TI>::t* __restrict__ conv,
     {
         const int s = support[p];
         const int W = 2 * s + 1;
-        const unsigned tab_len = (unsigned)W * W;
+        const unsigned tab_len = (unsigned)(W + 2 * kTablePadRows) * W;
         const unsigned total = tab_len * noff * noff;
         const typename V2<TI>::t* src = conv + plane_len * p;
         typename V2<TA>::t* dst = tables + plane_off[p];
@@ -52,14 +54,20 @@ __global__ void k_expand_tables(const typename V2<TI>::t* __restrict__ conv,
         {
             const unsigned tab = e / tab_len;
             const unsigned rem = e - tab * tab_len;
-            const int jj = (int)(rem / W), kk = (int)(rem - (unsigned)jj * W);
-            const int a = (int)tab / noff, b = (int)tab - ((int)tab / noff) * noff;
-            const int row = abs(b + oversample * (jj - s));
-            const int col = abs(a + oversample * (kk - s));
-            const typename V2<TI>::t v = src[(size_t)row * conv_size_half + col];
+            const int jj = (int)(rem / W) - kTablePadRows;
+            const int kk = (int)(rem % W);
             typename V2<TA>::t o;
-            o.x = (TA)v.x;
-            o.y = (TA)v.y;
+            o.x = (TA)0;
+            o.y = (TA)0;
+            if (jj >= 0 && jj < W)
+            {
+                const int a = (int)tab / noff, b = (int)tab - ((int)tab / noff) * noff;
+                const int row = abs(b + oversample * (jj - s));
+                const int col = abs(a + oversample * (kk - s));
+                const typename V2<TI>::t v = src[(size_t)row * conv_size_half + col];
+                o.x = (TA)v.x;
+                o.y = (TA)v.y;
+            }
             dst[e] = o;
         }
     }
@@ -116,6 +124,7 @@ struct SelectArgs
     int num_planes; const int* support; int oversample; int half_ov;
     const double* norm_table; const unsigned* plane_off;
     int table_id_shift;              // table id >> shift fits kTableIdBits
+    int tw, th;                      // bucketing tile width / height (cells)
     // outputs
     int4* meta; typename V4<TA>::t* amp; unsigned* num_visits;
     double* part_norm; unsigned long long* part_skipped;
@@ -233,14 +242,17 @@ void k_select(const SelectArgs<TI, TA> A, unsigned num_chunks)
             }
             m.z = (int)(((table_id >> A.table_id_shift) << kTableIdShift) |
                     ((unsigned)s.support << kSupportShift) | (unsigned)flags);
-            m.w = (int)(A.plane_off[s.plane] + tab * (unsigned)W * (unsigned)W);
+            m.w = (int)(A.plane_off[s.plane] + tab * (unsigned)(W + 2 * kTablePadRows) * (unsigned)W +
+                    (unsigned)(kTablePadRows * W));
             A.meta[i] = m;
             // Tile visits (none for non-finite amplitudes: k_grid_nonfinite
             // grids those after the tiles are done).
-            const int tx0 = (s.grid_u - s.support) >> kLogTileW;
-            const int tx1 = (s.grid_u + s.support) >> kLogTileW;
-            const int ty0 = (s.grid_v - s.support) >> kLogTileH;
-            const int ty1 = (s.grid_v + s.support) >> kLogTileH;
+            // (footprints of gridded visibilities lie inside the grid: the
+            // operands are non-negative)
+            const int tx0 = (s.grid_u - s.support) / A.tw;
+            const int tx1 = (s.grid_u + s.support) / A.tw;
+            const int ty0 = (s.grid_v - s.support) / A.th;
+            const int ty1 = (s.grid_v + s.support) / A.th;
             A.num_visits[i] = (flags & kFlagNonFinite) ? 0u
                     : (unsigned)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
             // Norm term (oskar_grid_wproj.c:210), taps, bounding box.
@@ -488,7 +500,7 @@ __global__ __launch_bounds__(256)
 void k_emit(const int4* __restrict__ meta, const unsigned* __restrict__ num_visits,
         const unsigned* __restrict__ visit_off, const JobDev<T>* __restrict__ jobs,
         const unsigned long long* __restrict__ job_vis_off, int num_jobs,
-        unsigned long long n, int key_table_shift, int key_table_bits,
+        unsigned long long n, int tw, int th, int key_table_shift, int key_table_bits,
         unsigned* __restrict__ keys, unsigned* __restrict__ vals)
 {
     for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
@@ -506,9 +518,11 @@ void k_emit(const int4* __restrict__ meta, const unsigned* __restrict__ num_visi
         const unsigned tile_base = jobs[lo].tile_base;
         const int4 m = meta[i];
         const int s = meta_support(m.z);
-        const unsigned kappa = meta_table_id(m.z) >> key_table_shift;
-        const int tx0 = (m.x - s) >> kLogTileW, tx1 = (m.x + s) >> kLogTileW;
-        const int ty0 = (m.y - s) >> kLogTileH, ty1 = (m.y + s) >> kLogTileH;
+        // kernel table, then its orientation (the two mirror flags)
+        const unsigned kappa = ((meta_table_id(m.z) << 2) | (unsigned)(m.z & kOrientMask)) >>
+                key_table_shift;
+        const int tx0 = (m.x - s) / tw, tx1 = (m.x + s) / tw;
+        const int ty0 = (m.y - s) / th, ty1 = (m.y + s) / th;
         unsigned o = visit_off[i];
         for (int ty = ty0; ty <= ty1; ++ty)
             for (int tx = tx0; tx <= tx1; ++tx, ++o)
@@ -643,10 +657,10 @@ __global__ void k_tile_bounds(const unsigned* __restrict__ keys,
     }
 }
 
-// Runs: maximal stretches of sorted visits with the same tile and the same
-// kernel table.  flag[p] = 1 where a run starts.  When the sort key holds
-// the full table id, key equality decides; otherwise the table bases are
-// compared through the visit records.
+// Runs: maximal stretches of sorted visits with the same tile, the same
+// kernel table and the same orientation.  flag[p] = 1 where a run starts.
+// When the sort key holds the full table id and orientation, key equality
+// decides; otherwise the visit records are compared.
 __global__ void k_run_flags(const unsigned* __restrict__ keys,
         const unsigned* __restrict__ vals, const int4* __restrict__ meta,
         const unsigned long long* __restrict__ d_n, int exact_key,
@@ -657,7 +671,11 @@ __global__ void k_run_flags(const unsigned* __restrict__ keys,
             p < n; p += (unsigned long long)gridDim.x * blockDim.x)
     {
         unsigned f = p == 0 || keys[p] != keys[p - 1];
-        if (!f && !exact_key) f = meta[vals[p]].w != meta[vals[p - 1]].w;
+        if (!f && !exact_key)
+        {
+            const int4 a = meta[vals[p]], b = meta[vals[p - 1]];
+            f = a.w != b.w || ((a.z ^ b.z) & kOrientMask);
+        }
         flag[p] = f;
     }
 }
@@ -737,6 +755,10 @@ struct GridArgs
     const unsigned* run_start; const int* run_z; const int* run_tb;
     const unsigned long long* d_num_runs;
     unsigned min_stage_run;        // shortest run staged through LDS
+    // byte sizes for raw buffer access (strip kernel)
+    unsigned meta_bytes;           // meta (and amp) records, < 2^32
+    unsigned visit_bytes;          // visit_vis
+    unsigned run_bytes;            // run_start / run_z / run_tb allocations
 };
 
 // TwoSum (Knuth): s + e == a + b exactly.
@@ -967,17 +989,25 @@ void k_grid_tiles(const GridArgs<T> A, unsigned grid_padded)
 }
 
 // Ordered merge: for each tile split over n >= 2 items, grid += sum of the
-// item partials, summed in item order in double.
+// item partials, summed in item order in double.  Block (c, t): tile t's
+// cells 1024c .. 1024c+1023, four per thread; the partial loads of up to
+// four items are in flight together, the sums stay in item order.
+constexpr int kMergeCells = 1024;
+
 template <typename T, typename P2>
 __global__ __launch_bounds__(256)
 void k_merge(const unsigned* __restrict__ item_off, const unsigned* __restrict__ slot_off,
         unsigned num_tiles, const JobDev<T>* __restrict__ jobs,
-        const unsigned* __restrict__ job_tile_base, int num_jobs,
+        const unsigned* __restrict__ job_tile_base, int num_jobs, int tw, int th,
         const P2* __restrict__ scratch)
 {
     typedef typename V2<T>::t T2;
-    for (unsigned t = blockIdx.x; t < num_tiles; t += gridDim.x)
+    const int cells = tw * th;   // a multiple of kMergeCells
+    const unsigned chunks = (unsigned)(cells / kMergeCells);
+    for (unsigned long long w = blockIdx.x; w < (unsigned long long)num_tiles * chunks; w += gridDim.x)
     {
+        const unsigned t = (unsigned)(w / chunks);
+        const unsigned ch = (unsigned)(w - (unsigned long long)t * chunks);
         const unsigned nit = item_off[t + 1] - item_off[t];
         if (nit < 2) continue;
         int lo = 0, hi = num_jobs - 1;
@@ -991,24 +1021,36 @@ void k_merge(const unsigned* __restrict__ item_off, const unsigned* __restrict__
         const unsigned local = t - J.tile_base;
         const int ty = (int)(local / (unsigned)J.ntx);
         const int tx = (int)(local - (unsigned)ty * J.ntx);
-        const unsigned s0 = slot_off[t];
-        for (int c = threadIdx.x; c < kTileCells; c += 256)
+        const P2* part = scratch + (size_t)slot_off[t] * cells;
+        #pragma unroll
+        for (int k = 0; k < kMergeCells / 256; ++k)
         {
-            const int x = tx * kTileW + (c & (kTileW - 1));
-            const int y = ty * kTileH + c / kTileW;
+            const int c = (int)ch * kMergeCells + k * 256 + (int)threadIdx.x;
             double re = 0.0, im = 0.0;
-            for (unsigned q = 0; q < nit; ++q)
+            unsigned q = 0;
+            for (; q + 4 <= nit; q += 4)
             {
-                const P2 v = scratch[(size_t)(s0 + q) * kTileCells + c];
+                P2 v[4];
+                #pragma unroll
+                for (int u = 0; u < 4; ++u) v[u] = part[(size_t)(q + u) * cells + c];
+                #pragma unroll
+                for (int u = 0; u < 4; ++u) { re += (double)v[u].x; im += (double)v[u].y; }
+            }
+            for (; q < nit; ++q)
+            {
+                const P2 v = part[(size_t)q * cells + c];
                 re += (double)v.x; im += (double)v.y;
             }
-            if (x < G && y < G)
+            const int y = c / tw;
+            const int x = tx * tw + (c - y * tw);
+            const int yg = ty * th + y;
+            if (x < G && yg < G)
             {
-                T2* g = J.grid + (size_t)y * G + x;
-                T2 v = *g;
-                v.x = (T)((double)v.x + re);
-                v.y = (T)((double)v.y + im);
-                *g = v;
+                T2* g = J.grid + (size_t)yg * G + x;
+                T2 gv = *g;
+                gv.x = (T)((double)gv.x + re);
+                gv.y = (T)((double)gv.y + im);
+                *g = gv;
             }
         }
     }

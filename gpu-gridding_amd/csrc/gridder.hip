@@ -6,6 +6,7 @@
 // device-to-host reads (visit total, work-item total) to size buffers.
 #include "gg_kernels.hpp"
 #include "gg_grid_staged.hpp"
+#include "gg_grid_strips.hpp"
 #include "mi355_gridder.h"
 
 #include <algorithm>
@@ -132,8 +133,8 @@ struct mi355_ctx
     bool compensated = false;
     bool timing = false;
     bool cache_kernels = false;
-    int grid_kernel = 0;            // 0 staged, 1 direct
-    unsigned min_stage_run = 2;
+    int grid_kernel = 0;            // 0 auto (strips for plain FP32), 1 direct 64x64, 2 staged 64x64
+    unsigned min_stage_run = 4;
 
     // prepared kernel tables
     const void* cache_conv = nullptr;
@@ -141,7 +142,8 @@ struct mi355_ctx
     std::vector<int> cache_support;
     bool tables_valid = false;
     std::vector<unsigned> plane_off;
-    unsigned long long table_elems = 0;
+    unsigned long long table_elems = 0;   // guard + tables (the zero tail follows)
+    unsigned long long table_tail = 0;
     int max_support = 0;
 
     DevBuf tables, norm_table, support_d, plane_off_d;
@@ -262,28 +264,41 @@ int prepare_tables(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
         return MI355_OK;
     C->tables_valid = false;
     const int half = (ov + 1) / 2, noff = half + 1;
-    C->plane_off.resize(P);
-    unsigned long long off = 0;
     int smax = 0;
+    for (size_t p = 0; p < P; ++p) smax = std::max(smax, support[p]);
+    // Layout: guard rows, then per plane noff^2 padded tables (see
+    // k_expand_tables), then the zero tail (guard rows and the zero row the
+    // 64-column kernels read for rows outside a footprint).  32-bit byte
+    // offsets, below 2^31 so that an offset of 2^31 is out of range.
+    const unsigned long long Wmax = 2ull * smax + 1;
+    const unsigned long long guard = (unsigned long long)kTableGuardRows * Wmax;
+    C->plane_off.resize(P);
+    unsigned long long off = guard;
     for (size_t p = 0; p < P; ++p)
     {
         C->plane_off[p] = (unsigned)off;
         const unsigned long long W = 2ull * support[p] + 1;
-        off += W * W * noff * noff;
-        if (off > 0xFFFFFFFFull)
-            return fail(MI355_ERR_TOO_LARGE, "expanded kernel tables exceed 2^32 elements");
-        smax = std::max(smax, support[p]);
+        off += (W + 2 * kTablePadRows) * W * noff * noff;
+        if (off > 0x7FFFFFFFull)
+            return fail(MI355_ERR_TOO_LARGE, "expanded kernel tables exceed 2^31 elements");
     }
     C->table_elems = off;
     C->max_support = smax;
     typedef typename V2<TA>::t T2;
-    // The tables are followed by a zero row read for rows outside a
-    // footprint; the whole buffer is addressed with 32-bit byte offsets.
-    const unsigned long long zero_elems = 2ull * kMaxSupport + 1;
-    if ((off + zero_elems) * sizeof(T2) > 0xFFFFFFFFull)
-        return fail(MI355_ERR_TOO_LARGE, "expanded kernel tables exceed 4 GiB");
-    TRY(ensure(C->tables, (off + zero_elems) * sizeof(T2)));
-    HIP_TRY(hipMemsetAsync(C->tables.as<T2>() + off, 0, zero_elems * sizeof(T2), C->stream));
+    const unsigned long long tail = std::max(guard, 2ull * kMaxSupport + 1);
+    if ((off + tail) * sizeof(T2) > 0x7FFFFFFFull)
+        return fail(MI355_ERR_TOO_LARGE, "expanded kernel tables exceed 2 GiB");
+    TRY(ensure(C->tables, (off + tail) * sizeof(T2)));
+    C->table_tail = tail;
+    {
+        // zero the guard and the tail (fill kernels: no memset nodes)
+        const unsigned wpe = sizeof(T2) / 4;
+        hipLaunchKernelGGL(k_fill_u32, dim3(grid_blocks(guard * wpe, 256, 1024)), dim3(256), 0,
+                C->stream, C->tables.as<unsigned>(), guard * wpe, 0u);
+        hipLaunchKernelGGL(k_fill_u32, dim3(grid_blocks(tail * wpe, 256, 1024)), dim3(256), 0,
+                C->stream, reinterpret_cast<unsigned*>(C->tables.as<T2>() + off), tail * wpe, 0u);
+        HIP_TRY(hipGetLastError());
+    }
     TRY(ensure(C->norm_table, P * (2 * half + 1) * (2 * half + 1) * sizeof(double)));
     TRY(ensure(C->support_d, P * sizeof(int)));
     TRY(ensure(C->plane_off_d, P * sizeof(unsigned)));
@@ -293,8 +308,7 @@ int prepare_tables(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     HIP_TRY(hipMemcpyAsync(C->plane_off_d.p, C->plane_off.data(), P * sizeof(unsigned),
             hipMemcpyHostToDevice, C->stream));
     {
-        const unsigned long long Wm = 2ull * smax + 1;
-        const unsigned per_plane = (unsigned)(Wm * Wm * noff * noff);
+        const unsigned per_plane = (unsigned)((Wmax + 2 * kTablePadRows) * Wmax * noff * noff);
         dim3 g(grid_blocks(per_plane, 256, 1024), (unsigned)std::min<size_t>(P, 65535));
         hipLaunchKernelGGL((k_expand_tables<TI, TA>), g, dim3(256), 0, C->stream,
                 conv_dev, (int)P, csh, ov, half, C->support_d.as<int>(),
@@ -390,6 +404,18 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     TRY((prepare_tables<TI, TA>(C, P, support, ov, csh, conv)));
     record(C, EV_PREP);
     const int half = (ov + 1) / 2, noff = half + 1;
+    // Gridding kernel and bucketing geometry: FP32 without compensation uses
+    // the strip kernel on 256 x 128 regions unless another kernel is asked
+    // for (or the call is too large for its 32-bit record offsets); the
+    // 64 x 64 kernels take every other case.
+    const bool comp = C->compensated;
+    unsigned long long n_call = 0;
+    for (int j = 0; j < num_jobs; ++j) n_call += jobs[j].num_points;
+    const bool strips = std::is_same<T, float>::value && !comp && C->grid_kernel == 0 &&
+            n_call < (1ull << 28);
+    const bool direct64 = !strips && C->grid_kernel == 1;
+    const int tw = strips ? kRegionW : kTileW;
+    const int th = strips ? kRegionH : kTileH;
 
     // Jobs.
     C->h_jobs.resize(sizeof(JobDev<T>) * num_jobs);
@@ -416,8 +442,8 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
         d.grid_scale_d = (double)gs;
         d.w_scale_d = (double)J.w_scale;
         d.grid_size = J.grid_size;
-        d.ntx = (J.grid_size + kTileW - 1) / kTileW;
-        d.nty = (J.grid_size + kTileH - 1) / kTileH;
+        d.ntx = (J.grid_size + tw - 1) / tw;
+        d.nty = (J.grid_size + th - 1) / th;
         d.tile_base = (unsigned)tiles;
         d.vis_begin = n_total;
         d.vis_end = n_total + J.num_points;
@@ -441,10 +467,13 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     const int tile_bits = std::max(1, bits_for(tiles));
     if (tile_bits > 32) return fail(MI355_ERR_TOO_LARGE, "tile id exceeds 32 bits");
     const unsigned long long num_tables = (unsigned long long)P * noff * noff;
+    // Sort key: tile, then the table id as the visit records hold it, then
+    // the two orientation bits; low bits are dropped if it exceeds 32 bits.
     const int table_bits_full = bits_for(num_tables);
     const int table_id_shift = std::max(0, table_bits_full - kTableIdBits);
-    const int key_table_bits = std::min(std::max(0, table_bits_full - table_id_shift), 32 - tile_bits);
-    const int key_table_shift = std::max(0, table_bits_full - table_id_shift - key_table_bits);
+    const int kappa_bits = table_bits_full - table_id_shift + 2;
+    const int key_table_bits = std::min(kappa_bits, 32 - tile_bits);
+    const int key_table_shift = kappa_bits - key_table_bits;
 
     TRY(ensure(C->jobs_d, C->h_jobs.size()));
     TRY(ensure(C->job_chunk_off_d, (num_jobs + 1) * sizeof(unsigned)));
@@ -481,6 +510,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     sa.norm_table = C->norm_table.as<double>();
     sa.plane_off = C->plane_off_d.as<unsigned>();
     sa.table_id_shift = table_id_shift;
+    sa.tw = tw; sa.th = th;
     sa.meta = C->meta.as<int4>(); sa.amp = C->amp.as<T4>();
     sa.num_visits = C->num_visits.as<unsigned>();
     sa.part_norm = C->part_norm.as<double>();
@@ -543,7 +573,8 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     hipLaunchKernelGGL(k_emit<T>, dim3(grid_blocks(n_total, 256, 16384)), dim3(256), 0, C->stream,
             C->meta.as<int4>(), C->num_visits.as<unsigned>(), C->visit_off.as<unsigned>(),
             C->jobs_d.as<JobDev<T>>(), C->job_vis_off_d.as<unsigned long long>(), num_jobs,
-            n_total, key_table_shift, key_table_bits, C->keys[0].as<unsigned>(), C->vals[0].as<unsigned>());
+            n_total, tw, th, key_table_shift, key_table_bits, C->keys[0].as<unsigned>(),
+            C->vals[0].as<unsigned>());
     HIP_TRY(hipGetLastError());
     const int key_bits = tile_bits + key_table_bits;
     const unsigned long long nchunk = (T_vis + kSortChunk - 1) / kSortChunk;
@@ -567,12 +598,13 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     const unsigned* skeys = C->keys[cur].as<unsigned>();
     const unsigned* svals = C->vals[cur].as<unsigned>();
 
-    // Runs of visits sharing a tile and a kernel table (staged kernel only):
-    // flags -> exclusive scan -> per-run start, packed word and table base.
-    // Flags and scan live in the sort's free ping-pong buffers.
+    // Runs of visits sharing a tile, a kernel table and an orientation (the
+    // LDS-staging kernels): flags -> exclusive scan -> per-run start, packed
+    // word and table base.  Flags and scan live in the sort's free ping-pong
+    // buffers.
     unsigned* run_flag = C->keys[cur ^ 1].as<unsigned>();
     unsigned* run_excl = C->vals[cur ^ 1].as<unsigned>();
-    if (C->grid_kernel == 0)
+    if (!direct64)
     {
         const int exact_key = table_id_shift == 0 && key_table_shift == 0;
         const unsigned vb = grid_blocks(T_vis, 256, 65535);
@@ -624,12 +656,11 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     C->stats.num_items = num_items;
 
     // Grid.
-    const bool comp = C->compensated;
     const size_t part_bytes = comp ? sizeof(double2) : sizeof(T2);
-    TRY(ensure(C->scratch, num_slots * kTileCells * part_bytes));
+    TRY(ensure(C->scratch, num_slots * (unsigned long long)(tw * th) * part_bytes));
     GridArgs<T> ga;
     ga.tables = C->tables.as<T2>();
-    ga.table_bytes = (unsigned)((C->table_elems + 2ull * kMaxSupport + 1) * sizeof(T2));
+    ga.table_bytes = (unsigned)((C->table_elems + C->table_tail) * sizeof(T2));
     ga.zero_row_byte = (unsigned)(C->table_elems * sizeof(T2));
     ga.meta = C->meta.as<int4>();
     ga.amp = C->amp.as<T4>();
@@ -652,8 +683,21 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     ga.run_tb = C->run_tb.as<int>();
     ga.d_num_runs = sc + SC_RUNS;
     ga.min_stage_run = C->min_stage_run;
+    ga.meta_bytes = (unsigned)std::min<unsigned long long>(n_total * 16ull, 0xFFFFFFFFull);
+    ga.visit_bytes = (unsigned)std::min<unsigned long long>(T_vis * 4ull, 0xFFFFFFFFull);
+    ga.run_bytes = (unsigned)std::min<unsigned long long>((T_vis + 1) * 4ull, 0xFFFFFFFFull);
     const unsigned padded = (unsigned)((num_items + 7) & ~7ull);
-    if (num_items && C->grid_kernel == 0)
+    if (num_items && strips)
+    {
+        // One workgroup of 16 waves per work item, in item order.
+        if constexpr (std::is_same<T, float>::value)
+        {
+            hipLaunchKernelGGL(k_grid_strips, dim3((unsigned)num_items), dim3(kStripBlock), 0,
+                    C->stream, ga);
+            HIP_TRY(hipGetLastError());
+        }
+    }
+    else if (num_items && !direct64)
     {
         // One workgroup per work item, in item order (consecutive items of a
         // heavy tile land on different XCDs).
@@ -687,16 +731,18 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     record(C, EV_GRID);
     if (num_slots)
     {
+        const unsigned merge_blocks = grid_blocks(
+                (unsigned long long)num_tiles * (unsigned)(tw * th / kMergeCells), 1, 1u << 20);
         if (comp)
-            hipLaunchKernelGGL((k_merge<T, double2>), dim3(std::min(num_tiles, 65535u)), dim3(256), 0, C->stream,
+            hipLaunchKernelGGL((k_merge<T, double2>), dim3(merge_blocks), dim3(256), 0, C->stream,
                     C->item_off.as<unsigned>(), C->slot_off.as<unsigned>(), num_tiles,
                     C->jobs_d.as<JobDev<T>>(), C->job_tile_base_d.as<unsigned>(), num_jobs,
-                    C->scratch.as<double2>());
+                    tw, th, C->scratch.as<double2>());
         else
-            hipLaunchKernelGGL((k_merge<T, T2>), dim3(std::min(num_tiles, 65535u)), dim3(256), 0, C->stream,
+            hipLaunchKernelGGL((k_merge<T, T2>), dim3(merge_blocks), dim3(256), 0, C->stream,
                     C->item_off.as<unsigned>(), C->slot_off.as<unsigned>(), num_tiles,
                     C->jobs_d.as<JobDev<T>>(), C->job_tile_base_d.as<unsigned>(), num_jobs,
-                    C->scratch.as<T2>());
+                    tw, th, C->scratch.as<T2>());
         HIP_TRY(hipGetLastError());
     }
     record(C, EV_MERGE);
@@ -1042,7 +1088,7 @@ MI355_EXPORT int mi355_ctx_set_option(mi355_ctx* C, int key, long long value)
     case MI355_OPT_PHASE_TIMING: C->timing = value != 0; break;
     case MI355_OPT_CACHE_KERNELS: C->cache_kernels = value != 0; C->tables_valid = false; break;
     case MI355_OPT_GRID_KERNEL:
-        if (value < 0 || value > 1) return fail(MI355_ERR_INVALID_ARG, "grid kernel %lld", value);
+        if (value < 0 || value > 2) return fail(MI355_ERR_INVALID_ARG, "grid kernel %lld", value);
         C->grid_kernel = (int)value;
         break;
     case MI355_OPT_MIN_STAGE_RUN:

@@ -96,8 +96,11 @@ MI355_EXPORT int mi355_ctx_set_stream(mi355_ctx* ctx, void* hip_stream);
 #define MI355_OPT_PHASE_TIMING    3  /* 1: record hipEvents around each phase */
 #define MI355_OPT_CACHE_KERNELS   4  /* 1: reuse prepared kernel tables while the
                                         conv_func pointer and shape are unchanged */
-#define MI355_OPT_GRID_KERNEL     5  /* 0: LDS-staged tables (default), 1: direct */
-#define MI355_OPT_MIN_STAGE_RUN   6  /* shortest same-table run staged in LDS (default 2) */
+#define MI355_OPT_GRID_KERNEL     5  /* 0: automatic (default): 16-column strips on
+                                        256x128 regions for plain FP32, else 2;
+                                        1: 64x64 tiles, tables read directly;
+                                        2: 64x64 tiles, tables staged in LDS */
+#define MI355_OPT_MIN_STAGE_RUN   6  /* shortest same-table run staged in LDS (default 4) */
 MI355_EXPORT int mi355_ctx_set_option(mi355_ctx* ctx, int key, long long value);
 
 /* Figures of the last gridding call on this context. */

@@ -184,6 +184,32 @@ def test_device_grid_against_fp64_oracle(gpu, cfg, N, G, P, smax):
     assert ec[0] <= 1e-5 and ec[1] <= el2 * 1.01 + 1e-9
 
 
+# Every gridding kernel: 16-column strips with tables staged for every run
+# that fits (min run 1), with no run staged (direct reads only), and the
+# 64 x 64 staged / direct kernels.  Bar as above: <= 1e-5 vs the FP64 truth.
+KERNELS = [(0, 1), (0, 1 << 30), (0, 4), (2, 2), (1, 2)]
+
+
+@pytest.mark.parametrize("kernel,min_run", KERNELS)
+def test_kernel_variants_against_fp64_oracle(gpu, kernel, min_run):
+    torch, ops = gpu
+    c = small_config(2, 30_000, 1024, 80, 72, seed=7)
+    t, tn, ts = oracle.grid_fsel_d(c)
+    ctx = ops.context()
+    ctx.set_option(_lib.OPT_GRID_KERNEL, kernel)
+    ctx.set_option(_lib.OPT_MIN_STAGE_RUN, min_run)
+    try:
+        g, n, s = run_grid(torch, ops, c, ctx=ctx)
+        g2, _, _ = run_grid(torch, ops, c, ctx=ctx, visits_per_item=53)
+    finally:
+        ctx.set_option(_lib.OPT_GRID_KERNEL, 0)
+        ctx.set_option(_lib.OPT_MIN_STAGE_RUN, 4)
+    assert s == ts and abs(n - tn) <= 1e-12 * abs(tn)
+    emax, el2 = errors(g, t)
+    assert emax <= 1e-5 and el2 <= 1e-5, (kernel, min_run, emax, el2)
+    assert errors(g2, t)[0] <= 1e-5
+
+
 def test_device_fp64_path_against_oracle_d(gpu):
     torch, ops = gpu
     c = small_config(0, 20_000, 1024)
