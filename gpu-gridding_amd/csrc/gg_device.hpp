@@ -37,6 +37,11 @@ constexpr int kSupportShift = 5;
 constexpr int kTableIdShift = 17;
 constexpr int kTableIdBits = 15;
 constexpr int kOrientMask = kFlagNegV | kFlagNegU;   // mirroring of a table read
+// In the sorted visit records of the strip kernel (never skipped, never
+// non-finite) the skip bit marks a visit whose run is staged in LDS and the
+// non-finite bit the first visit of a run.
+constexpr int kVisitStaged = kFlagSkip;
+constexpr int kVisitRunStart = kFlagNonFinite;
 // Expanded kernel tables carry this many zero rows above and below each
 // (2s+1)^2 table, so that reads of up to that many rows past a footprint
 // edge return zero taps.

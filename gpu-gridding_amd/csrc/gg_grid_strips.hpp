@@ -1,6 +1,6 @@
 // gg_grid_strips.hpp -- FP32 region gridding, one wave per 16-column strip.
 //
-// A workgroup of sixteen waves owns a 256 x 128 region of the grid; wave w
+// A workgroup of twelve waves owns a 192 x 128 region of the grid; wave w
 // owns the strip of columns 16w .. 16w+15 of it, all 128 rows.  A lane owns
 // one column of four rows of each of 32 slots (lane = 16 * row-in-slot +
 // column), so the strip's 2048 cells are 32 complex accumulators per lane,
@@ -18,8 +18,8 @@
 //
 // Rows go in chunks of eight (two slots).  A visit touches chunks q0..q1 of
 // the strip; a chunk is skipped with a wave-uniform branch.  From LDS the
-// taps of chunk q+1 are read while chunk q is multiplied; from global memory
-// the touched chunks are requested eight at a time before their multiplies.  Rows of a touched chunk
+// four slots of a pair of chunks are read together before their multiplies;
+// from global memory the touched chunks are requested kDirectGroup at a time.  Rows of a touched chunk
 // outside the footprint read zero rows, columns outside it zero columns
 // (staged) or nothing (direct: the buffer offset is out of range).
 //
@@ -35,15 +35,21 @@
 
 #include "gg_kernels.hpp"
 
-#include <type_traits>
-
 namespace gg {
 
 constexpr int kStripW = 16;                        // columns per wave
 constexpr int kStripRows = 128;                    // rows per wave
-constexpr int kStripWaves = 16;
-constexpr int kStripBlock = kStripWaves * 64;      // 1024 threads
-constexpr int kRegionW = kStripWaves * kStripW;    // 256 columns
+// Waves per workgroup (one workgroup per CU: the staged table takes the LDS)
+// and chunks of global taps loaded together.  Default: 12 waves (three per
+// SIMD, up to 168 VGPRs) with 8 chunks; at 16 waves (128 VGPRs) the 64
+// accumulators leave room for 1 chunk.
+#ifndef GG_STRIP_WAVES
+#define GG_STRIP_WAVES 12
+#endif
+constexpr int kStripWaves = GG_STRIP_WAVES;
+constexpr int kStripBlock = kStripWaves * 64;
+constexpr int kRegionW = kStripWaves * kStripW;    // 256 (or 192) columns
+constexpr int kStripWavesPerSimd = kStripWaves / 4;
 constexpr int kRegionH = kStripRows;               // 128 rows
 static_assert(kRegionW * kRegionH % kMergeCells == 0, "merge blocks tile a region");
 constexpr int kStripSlots = kStripRows / 4;        // 32 slots of 16 columns x 4 rows
@@ -51,13 +57,21 @@ constexpr int kStripChunks = kStripSlots / 2;      // 16 chunks of 8 rows
 constexpr int kStripS = 144;      // LDS row stride in taps; 144 = 16 (mod 32): the two
                                   // 128-byte row segments of a half-wave use disjoint banks
 constexpr int kStripPad = 7;      // zero rows above and below a staged table
-constexpr int kStripGuard = 16;   // zero taps before the first staged row
+// Staged row 0 starts at element kStripGuard.  Reads of a touched pair of
+// chunks cover rows -8 .. W+21 and columns -15 .. W+14 of the staged
+// layout: the eight rows before row 0 are inside the array (their taps are
+// never multiplied), the 16 taps right before row 0 are zero (columns -15..-1
+// of row 0 are multiplied).
+constexpr int kStripGuard = 8 * kStripS + 16;
 constexpr int kStripLds = 20480;  // float2 taps: 160 KiB
-constexpr int kStripMaxW = 119;   // widest table staged
-// Staged reads stay inside rows 0 .. W+21 (the chunk after the last one
-// touched is read ahead) and columns -15 .. W+14 of the staged layout.
+constexpr int kStripMaxW = 111;   // widest table staged
+#ifndef GG_DIRECT_GROUP
+#define GG_DIRECT_GROUP (GG_STRIP_WAVES >= 16 ? 1 : 8)
+#endif
+constexpr int kDirectGroup = GG_DIRECT_GROUP;   // chunks of global taps loaded together
+static_assert(kStripGuard - 8 * kStripS - 15 >= 0, "rows -8.. must stay inside the LDS array");
 static_assert(kStripGuard + (kStripMaxW + 21) * kStripS + kStripMaxW + 15 < kStripLds,
-              "read-ahead of the widest staged table must stay inside the LDS array");
+              "reads of the widest staged table must stay inside the LDS array");
 static_assert(kStripS - 15 >= kStripMaxW, "columns -15..-1 must land on zero columns");
 static_assert(kTablePadRows >= 7, "direct reads of a touched chunk go 7 rows past a table");
 // Direct read-ahead goes up to 15 rows past a table: the table buffer has
@@ -136,39 +150,35 @@ __device__ __forceinline__ void mac_strip(float& re, float& im, const float2 c, 
     im = fmaf(c.y, a.w, im);
 }
 
-// Chunk loop over LDS taps: LOAD(q, dst0, dst1) fetches the taps of chunk q
-// (slots 2q, 2q+1).  Even chunks' taps live in (e0, e1), odd chunks' in
-// (o0, o1), so the read-ahead of chunk q+1 needs no copies.
-#define GG_STRIP_CHUNKS(cm, LOAD)                                              \
+// Chunk loop over LDS taps, two chunks (16 rows, four slots) per block:
+// the four slots are read together, then each touched chunk of the pair is
+// multiplied.  t0 / t1: this lane's taps of chunk 0 / chunk 8.
+#define GG_STRIP_PAIRS(cm)                                                     \
     {                                                                         \
-        const int qa_ = __builtin_ctz((unsigned)(cm));                        \
-        float2 e0, e1, o0, o1;                                                \
-        if (qa_ & 1) { LOAD(qa_, o0, o1); } else { LOAD(qa_, e0, e1); }       \
         _Pragma("unroll")                                                     \
-        for (int q = 0; q < kStripChunks; ++q)                                \
+        for (int pq = 0; pq < kStripChunks / 2; ++pq)                         \
         {                                                                     \
-            if ((cm) & (1 << q))                                              \
+            if (((cm) >> (2 * pq)) & 3)                                       \
             {                                                                 \
-                if (q + 1 < kStripChunks)                                     \
+                const float2* pp = pq < 4 ? t0 + pq * 16 * kStripS            \
+                                          : t1 + (pq - 4) * 16 * kStripS;     \
+                const float2 c0 = pp[0], c1 = pp[4 * kStripS];                \
+                const float2 c2 = pp[8 * kStripS], c3 = pp[12 * kStripS];     \
+                if ((cm) & (1 << (2 * pq)))                                   \
                 {                                                             \
-                    if ((q + 1) & 1) { LOAD(q + 1, o0, o1); }                 \
-                    else { LOAD(q + 1, e0, e1); }                             \
+                    mac_strip(re[4 * pq], im[4 * pq], c0, a);                 \
+                    mac_strip(re[4 * pq + 1], im[4 * pq + 1], c1, a);         \
                 }                                                             \
-                if (q & 1)                                                    \
+                if ((cm) & (2 << (2 * pq)))                                   \
                 {                                                             \
-                    mac_strip(re[2 * q], im[2 * q], o0, a);                   \
-                    mac_strip(re[2 * q + 1], im[2 * q + 1], o1, a);           \
-                }                                                             \
-                else                                                          \
-                {                                                             \
-                    mac_strip(re[2 * q], im[2 * q], e0, a);                   \
-                    mac_strip(re[2 * q + 1], im[2 * q + 1], e1, a);           \
+                    mac_strip(re[4 * pq + 2], im[4 * pq + 2], c2, a);         \
+                    mac_strip(re[4 * pq + 3], im[4 * pq + 3], c3, a);         \
                 }                                                             \
             }                                                                 \
         }                                                                     \
     }
 
-__global__ __launch_bounds__(kStripBlock) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ __launch_bounds__(kStripBlock) __attribute__((amdgpu_waves_per_eu(kStripWavesPerSimd)))
 void k_grid_strips(const GridArgs<float> A)
 {
     __shared__ float2 tab[kStripLds];
@@ -180,13 +190,7 @@ void k_grid_strips(const GridArgs<float> A)
     // visit's footprint starts at strip row 0, column 0.
     const int lane_lds = kStripGuard + (kStripPad + lr) * kStripS + lc;
     const __amdgpu_buffer_rsrc_t rs_tab = table_rsrc(A.tables, A.table_bytes);
-    const __amdgpu_buffer_rsrc_t rs_meta = table_rsrc(A.meta, A.meta_bytes);
-    const __amdgpu_buffer_rsrc_t rs_amp = table_rsrc(A.amp, A.meta_bytes);
-    const __amdgpu_buffer_rsrc_t rs_vis = table_rsrc(A.visit_vis, A.visit_bytes);
-    const __amdgpu_buffer_rsrc_t rs_rstart = table_rsrc(A.run_start, A.run_bytes);
-    const __amdgpu_buffer_rsrc_t rs_rz = table_rsrc(A.run_z, A.run_bytes);
-    const __amdgpu_buffer_rsrc_t rs_rtb = table_rsrc(A.run_tb, A.run_bytes);
-    if (threadIdx.x < kStripGuard) tab[threadIdx.x] = make_float2(0.f, 0.f);
+    if (threadIdx.x < 16) tab[kStripGuard - 16 + threadIdx.x] = make_float2(0.f, 0.f);
     int staged_base = -1, staged_orient = -1;      // table in LDS (uniform)
 
     for (unsigned item = blockIdx.x; item < num_items; item += gridDim.x)
@@ -219,169 +223,163 @@ void k_grid_strips(const GridArgs<float> A)
         #pragma unroll
         for (int i = 0; i < kStripSlots; ++i) re[i] = im[i] = 0.f;
 
-        // Two passes over the item's visits: first the visits of runs read
-        // from global memory (no barriers: waves run at their own pace), then
-        // the runs staged in LDS.  Runs' records are fetched 64 at a time
-        // (lane l: run rbase + l).
-        auto walk = [&](auto staged_pass) {
-            constexpr bool STAGED = decltype(staged_pass)::value;
-            unsigned rbase = ld_const(A.run_excl, v0) + ld_const(A.run_flag, v0) - 1u;
-            unsigned rv_start, rv_next;
-            int rv_z, rv_tb;
-            auto load_runs = [&]() {
-                const unsigned o = (rbase + (unsigned)lane) * 4u;
-                rv_start = rb_u32(rs_rstart, o);
-                rv_next = rb_u32(rs_rstart, o + 4u);
-                rv_z = (int)rb_u32(rs_rz, o);
-                rv_tb = (int)rb_u32(rs_rtb, o);
-            };
-            load_runs();
-            int rl = 0;
+        // (resources based at the item's records keep buffer offsets small;
+        // reads past the item's last visit return zero)
+        const __amdgpu_buffer_rsrc_t rs_vmeta = table_rsrc(A.vmeta + v0, (v1 - v0) * 16u);
+        const __amdgpu_buffer_rsrc_t rs_vamp = table_rsrc(A.vamp + v0, (v1 - v0) * 16u);
 
-            for (unsigned b = v0; b < v1; b += 64)
+        // The visits of a batch of 64 (lane l: visit b + l) that this strip
+        // processes: those that meet its columns, of the wanted kind.
+        auto batch_mask = [&](unsigned b, unsigned e, int4& mv, float4& av, bool staged_kind) {
+            const unsigned pl = b + (unsigned)lane;
+            mv = rb_x4<int4>(rs_vmeta, (pl - v0) * 16u);
+            av = rb_x4<float4>(rs_vamp, (pl - v0) * 16u);
+            const int s = meta_support(mv.z);
+            return __ballot(pl < e && ((mv.z & kVisitStaged) != 0) == staged_kind &&
+                            mv.x - s <= xs + kStripW - 1 && mv.x + s >= xs);
+        };
+        // Per visit (scalar, from lane l of the batch): support, footprint
+        // origin relative to the strip (kc: footprint column of strip column
+        // 0; top: footprint row 0), chunks q0..q1 touched, coefficients.  (A
+        // visit of the region's list meets its rows: top <= 127, top+2s >= 0.)
+#define GG_VISIT_HEAD                                                           \
+        const int x = bcast(mv.x, l), y = bcast(mv.y, l), z = bcast(mv.z, l);   \
+        float4 a;                                                               \
+        a.x = bcast(av.x, l); a.y = bcast(av.y, l);                             \
+        a.z = bcast(av.z, l); a.w = bcast(av.w, l);                             \
+        const int s = meta_support(z);                                          \
+        const int kc = xs - (x - s);                                            \
+        const int top = y - s - ys;                                             \
+        const int q0 = max(top, 0) >> 3;                                        \
+        const int q1 = min(top + 2 * s, kStripRows - 1) >> 3;                   \
+        const int cm = (int)((2u << q1) - (1u << q0));
+
+        // Pass 1: visits of runs read straight from the tables in global
+        // memory.  No barriers: the waves run at their own pace.
+        for (unsigned b = v0; b < v1; b += 64)
+        {
+            int4 mv;
+            float4 av;
+            unsigned long long todo = batch_mask(b, v1, mv, av, false);
+            while (todo)
             {
-                // Records of visits b + l; which of them meet this strip.
-                const unsigned pl = b + (unsigned)lane;
-                const unsigned iv = rb_u32(rs_vis, pl * 4u);
-                const int4 mv = rb_x4<int4>(rs_meta, iv * 16u);
-                const float4 av = rb_x4<float4>(rs_amp, iv * 16u);
-                const unsigned long long meets_mask = __ballot(pl < v1 &&
-                        mv.x - meta_support(mv.z) <= xs + kStripW - 1 &&
-                        mv.x + meta_support(mv.z) >= xs);
-                const unsigned bend = min(b + 64u, v1);
-
-                for (unsigned p = b; p < bend;)
+                const int l = __builtin_ctzll(todo);
+                todo &= todo - 1ull;
+                GG_VISIT_HEAD
+                const int tb = bcast(mv.w, l);
+                const int W = 2 * s + 1;
+                const bool neg_v = z & kFlagNegV, neg_u = z & kFlagNegU;
+                // Lane's footprint column, its table column, and its row
+                // within a slot (rows run backwards when mirrored).
+                const int k = kc + lc;
+                const int kt = neg_u ? W - 1 - k : k;
+                const int rrow = neg_v ? 3 - lr : lr;
+                const unsigned voff = (k >= 0 && k < W) ? (unsigned)(rrow * W + kt) * 8u
+                                                        : 0x80000000u;
+                // Byte offset of chunk 0's first table row (the row of lane
+                // row 0, or row 3 when mirrored) and the steps.
+                const int row0 = neg_v ? (W - 1) - 3 + top : -top;
+                const int base = (tb + row0 * W) * 8;
+                const int step = (neg_v ? -64 : 64) * W;
+                const int half = step >> 1;
+                // Touched chunks' taps are requested kDirectGroup chunks at a time
+                // before their multiplies: up to 2 * kDirectGroup loads in flight.
+                #pragma unroll
+                for (int h = 0; h < kStripChunks; h += kDirectGroup)
                 {
-                    if (rl == 64)
+                    if (!((cm >> h) & ((1 << kDirectGroup) - 1))) continue;
+                    float2 tp[kDirectGroup][2];
+                    #pragma unroll
+                    for (int u = 0; u < kDirectGroup; ++u)
                     {
-                        rbase += 64;
-                        load_runs();
-                        rl = 0;
-                    }
-                    const unsigned rstart = (unsigned)bcast((int)rv_start, rl);
-                    const unsigned rnext = (unsigned)bcast((int)rv_next, rl);
-                    const int rtb = bcast(rv_tb, rl);
-                    const int rz = bcast(rv_z, rl);
-                    const int W0 = 2 * meta_support(rz) + 1;
-                    const int orient = rz & kOrientMask;
-                    // (max: every step advances, whatever the run table holds)
-                    const unsigned e = max(min(rnext, bend), p + 1);
-                    const bool stage_run = W0 <= kStripMaxW && rnext - rstart >= A.min_stage_run;
-                    if (e >= rnext) ++rl;      // the run ends in this batch
-                    const int lb = (int)(p - b), le = (int)(e - b);
-                    p = e;
-                    if (stage_run != STAGED) continue;
-                    if (STAGED && (rtb != staged_base || orient != staged_orient))
-                    {
-                        __syncthreads();
-                        stage_strip_table(tab, rs_tab, rtb, W0, orient, wave, lane);
-                        __syncthreads();
-                        staged_base = rtb;
-                        staged_orient = orient;
-                    }
-                    const unsigned long long seg = (le == 64 ? ~0ull : ((1ull << le) - 1ull)) &
-                            ~((1ull << lb) - 1ull);
-                    unsigned long long todo = meets_mask & seg;
-                    while (todo)
-                    {
-                        // Per visit (scalar): support, footprint origin relative
-                        // to the strip (kc: footprint column of strip column 0;
-                        // top: footprint row 0), chunks q0..q1 touched.
-                        const int l = __builtin_ctzll(todo);
-                        todo &= todo - 1ull;
-                        const int x = bcast(mv.x, l), y = bcast(mv.y, l), z = bcast(mv.z, l);
-                        float4 a;
-                        a.x = bcast(av.x, l); a.y = bcast(av.y, l);
-                        a.z = bcast(av.z, l); a.w = bcast(av.w, l);
-                        const int s = meta_support(z);
-                        const int kc = xs - (x - s);
-                        const int top = y - s - ys;
-                        // (a visit of the region's list meets its rows:
-                        // top <= 127 and top + 2s >= 0)
-                        const int q0 = max(top, 0) >> 3;
-                        const int q1 = min(top + 2 * s, kStripRows - 1) >> 3;
-                        const int cm = (int)((2u << q1) - (1u << q0));
-                        if constexpr (STAGED)
+                        if (cm & (1 << (h + u)))
                         {
-                            const float2* t0 = tab + (lane_lds + kc - top * kStripS);
-                            const float2* t1 = t0 + 64 * kStripS;
-#define GG_LDS_LOAD(qq, d0, d1)                                                 \
-                            {                                                   \
-                                const float2* pq = (qq) < 8 ? t0 + (qq) * 8 * kStripS \
-                                                            : t1 + ((qq) - 8) * 8 * kStripS; \
-                                d0 = pq[0];                                     \
-                                d1 = pq[4 * kStripS];                           \
-                            }
-                            GG_STRIP_CHUNKS(cm, GG_LDS_LOAD)
-#undef GG_LDS_LOAD
+                            const unsigned so = (unsigned)(base + (h + u) * step);
+                            tp[u][0] = load_tap(rs_tab, voff, so, (float2*)nullptr);
+                            tp[u][1] = load_tap(rs_tab, voff, so + (unsigned)half, (float2*)nullptr);
                         }
-                        else
+                    }
+                    #pragma unroll
+                    for (int u = 0; u < kDirectGroup; ++u)
+                    {
+                        const int q = h + u;
+                        if (cm & (1 << q))
                         {
-                            const int tb = bcast(mv.w, l);
-                            const int W = 2 * s + 1;
-                            const bool neg_v = z & kFlagNegV, neg_u = z & kFlagNegU;
-                            // Lane's footprint column, its table column, and its
-                            // row within a slot (rows run backwards when mirrored).
-                            const int k = kc + lc;
-                            const int kt = neg_u ? W - 1 - k : k;
-                            const int rrow = neg_v ? 3 - lr : lr;
-                            const unsigned voff = (k >= 0 && k < W) ? (unsigned)(rrow * W + kt) * 8u
-                                                                    : 0x80000000u;
-                            // Byte offset of chunk 0's first table row (the row
-                            // of lane row 0, or row 3 when mirrored), steps.
-                            const int row0 = neg_v ? (W - 1) - 3 + top : -top;
-                            const int base = (tb + row0 * W) * 8;
-                            const int step = (neg_v ? -64 : 64) * W;
-                            const int half = step >> 1;
-                            // Touched chunks' taps are requested eight chunks
-                            // at a time before their multiplies: up to 16
-                            // loads in flight.
-                            #pragma unroll
-                            for (int h = 0; h < kStripChunks; h += 8)
-                            {
-                                if (!((cm >> h) & 0xFF)) continue;
-                                float2 tp[8][2];
-                                #pragma unroll
-                                for (int u = 0; u < 8; ++u)
-                                {
-                                    if (cm & (1 << (h + u)))
-                                    {
-                                        const unsigned so = (unsigned)(base + (h + u) * step);
-                                        tp[u][0] = load_tap(rs_tab, voff, so, (float2*)nullptr);
-                                        tp[u][1] = load_tap(rs_tab, voff, so + (unsigned)half,
-                                                            (float2*)nullptr);
-                                    }
-                                }
-                                #pragma unroll
-                                for (int u = 0; u < 8; ++u)
-                                {
-                                    const int q = h + u;
-                                    if (cm & (1 << q))
-                                    {
-                                        mac_strip(re[2 * q], im[2 * q], tp[u][0], a);
-                                        mac_strip(re[2 * q + 1], im[2 * q + 1], tp[u][1], a);
-                                    }
-                                }
-                            }
+                            mac_strip(re[2 * q], im[2 * q], tp[u][0], a);
+                            mac_strip(re[2 * q + 1], im[2 * q + 1], tp[u][1], a);
                         }
                     }
                 }
             }
-        };
-        walk(std::false_type());
-        walk(std::true_type());
+        }
+
+        // Pass 2: visits of staged runs, 64 at a time.  The batch's staged
+        // visits split into segments at run starts (the first segment may
+        // continue a run of the previous batch); each segment's table is
+        // copied to LDS unless it is already there, then this strip's visits
+        // of the segment are processed.  Every wave sees the same segments,
+        // so the barriers around staging are uniform.
+        for (unsigned b = v0; b < v1; b += 64)
+        {
+            const unsigned pl = b + (unsigned)lane;
+            const int4 mv = rb_x4<int4>(rs_vmeta, (pl - v0) * 16u);
+            const float4 av = rb_x4<float4>(rs_vamp, (pl - v0) * 16u);
+            const bool stg = pl < v1 && (mv.z & kVisitStaged);
+            const unsigned long long stg_mask = __ballot(stg);
+            if (!stg_mask) continue;
+            const int sv = meta_support(mv.z);
+            const unsigned long long meets = __ballot(stg && mv.x - sv <= xs + kStripW - 1 &&
+                                                      mv.x + sv >= xs);
+            unsigned long long starts = __ballot(stg && (mv.z & kVisitRunStart)) |
+                    (stg_mask & (~stg_mask + 1ull));
+            while (starts)
+            {
+                const int l0 = __builtin_ctzll(starts);
+                starts &= starts - 1ull;
+                const unsigned long long upto = starts ? (starts & (~starts + 1ull)) - 1ull : ~0ull;
+                const int rtb = bcast(mv.w, l0);
+                const int rz = bcast(mv.z, l0);
+                const int orient = rz & kOrientMask;
+                if (rtb != staged_base || orient != staged_orient)
+                {
+                    __syncthreads();
+                    stage_strip_table(tab, rs_tab, rtb, 2 * meta_support(rz) + 1, orient, wave, lane);
+                    __syncthreads();
+                    staged_base = rtb;
+                    staged_orient = orient;
+                }
+                unsigned long long todo = meets & upto & ~((1ull << l0) - 1ull);
+                while (todo)
+                {
+                    const int l = __builtin_ctzll(todo);
+                    todo &= todo - 1ull;
+                    GG_VISIT_HEAD
+                    const float2* t0 = tab + (lane_lds + kc - top * kStripS);
+                    const float2* t1 = t0 + 64 * kStripS;
+                    GG_STRIP_PAIRS(cm)
+                }
+            }
+        }
+#undef GG_VISIT_HEAD
 
         // Strip out: lane (lr, lc) of slot i is cell (xs + lc, ys + 4i + lr).
+        // The lane's base offset goes through an empty asm so that the 32
+        // slot addresses are formed here rather than kept live (or spilled)
+        // from the start of the kernel.
         const int xo = xs + lc;
         if (nit == 1)
         {
+            unsigned long long lb = (unsigned long long)(ys + lr) * (unsigned)G + (unsigned)xo;
+            __asm__ volatile("" : "+v"(lb));
+            float2* g0 = J.grid + lb;
+            const size_t step4 = (size_t)4 * (unsigned)G;
             #pragma unroll
             for (int i = 0; i < kStripSlots; ++i)
             {
                 const int yo = ys + 4 * i + lr;
                 if (xo < G && yo < G)
                 {
-                    float2* g = J.grid + (size_t)yo * G + xo;
+                    float2* g = g0 + (size_t)i * step4;
                     float2 gv = *g;
                     gv.x += re[i];
                     gv.y += im[i];
@@ -392,17 +390,19 @@ void k_grid_strips(const GridArgs<float> A)
         else
         {
             const unsigned slot = ld_const(A.slot_off, t) + m;
+            unsigned lo = (unsigned)(lr * kRegionW + wave * kStripW + lc);
+            __asm__ volatile("" : "+v"(lo));
             float2* out = reinterpret_cast<float2*>(A.scratch) +
-                    (size_t)slot * (kRegionW * kRegionH);
+                    (size_t)slot * (kRegionW * kRegionH) + lo;
             #pragma unroll
             for (int i = 0; i < kStripSlots; ++i)
-                out[(4 * i + lr) * kRegionW + wave * kStripW + lc] = make_float2(re[i], im[i]);
+                out[i * 4 * kRegionW] = make_float2(re[i], im[i]);
         }
         // (the staged table stays valid for the next item: only staging
         // writes LDS, after a barrier)
     }
 }
 
-#undef GG_STRIP_CHUNKS
+#undef GG_STRIP_PAIRS
 
 } // namespace gg

@@ -705,6 +705,34 @@ __global__ void k_run_starts(const unsigned* __restrict__ flag,
     }
 }
 
+// Records of the sorted visits, in visit order (one load per batch of
+// visits in the strip kernel instead of an index load and a dependent one),
+// with kVisitStaged set where the visit's run is staged in LDS (at least
+// min_run visits and a table at most max_w wide) and kVisitRunStart where
+// the visit starts a run.
+template <typename T>
+__global__ void k_gather_visits(const unsigned* __restrict__ vals,
+        const int4* __restrict__ meta, const typename V4<T>::t* __restrict__ amp,
+        const unsigned* __restrict__ run_flag, const unsigned* __restrict__ run_excl,
+        const unsigned* __restrict__ run_start, unsigned min_run, int max_w,
+        const unsigned long long* __restrict__ d_n, int4* __restrict__ vmeta,
+        typename V4<T>::t* __restrict__ vamp)
+{
+    const unsigned long long n = *d_n;
+    for (unsigned long long p = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
+            p < n; p += (unsigned long long)gridDim.x * blockDim.x)
+    {
+        const unsigned i = vals[p];
+        int4 m = meta[i];
+        const unsigned r = run_excl[p] + run_flag[p] - 1u;
+        if (run_start[r + 1] - run_start[r] >= min_run && 2 * meta_support(m.z) + 1 <= max_w)
+            m.z |= kVisitStaged;
+        if (run_flag[p]) m.z |= kVisitRunStart;
+        vmeta[p] = m;
+        vamp[p] = amp[i];
+    }
+}
+
 // Work items per tile: ceil(visits / visits_per_item); tiles with two or more
 // items also take that many partial-tile slots.
 __global__ void k_items(const unsigned* __restrict__ tile_start,
@@ -758,6 +786,9 @@ struct GridArgs
     // byte sizes for raw buffer access (strip kernel)
     unsigned meta_bytes;           // meta (and amp) records, < 2^32
     unsigned visit_bytes;          // visit_vis
+    const int4* vmeta;             // meta / amp of each sorted visit (strip kernel)
+    const typename V4<T>::t* vamp;
+    unsigned vrec_bytes;           // unused
     unsigned run_bytes;            // run_start / run_z / run_tb allocations
 };
 

@@ -153,6 +153,7 @@ struct mi355_ctx
     DevBuf keys[2], vals[2], hist, hist_off;
     DevBuf tile_start, tile_end, n_items, n_slots, item_off, slot_off, item_tile;
     DevBuf run_start, run_z, run_tb;
+    DevBuf vmeta, vamp;             // sorted visit records (strip kernel)
     DevBuf scratch;
     DevBuf jobs_d, job_chunk_off_d, job_vis_off_d, job_tile_base_d, job_out_d, job_bbox_d;
     // drop-in staging
@@ -405,14 +406,10 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     record(C, EV_PREP);
     const int half = (ov + 1) / 2, noff = half + 1;
     // Gridding kernel and bucketing geometry: FP32 without compensation uses
-    // the strip kernel on 256 x 128 regions unless another kernel is asked
-    // for (or the call is too large for its 32-bit record offsets); the
-    // 64 x 64 kernels take every other case.
+    // the strip kernel on kRegionW x kRegionH regions unless another kernel is asked
+    // for; the 64 x 64 kernels take every other case.
     const bool comp = C->compensated;
-    unsigned long long n_call = 0;
-    for (int j = 0; j < num_jobs; ++j) n_call += jobs[j].num_points;
-    const bool strips = std::is_same<T, float>::value && !comp && C->grid_kernel == 0 &&
-            n_call < (1ull << 28);
+    const bool strips = std::is_same<T, float>::value && !comp && C->grid_kernel == 0;
     const bool direct64 = !strips && C->grid_kernel == 1;
     const int tw = strips ? kRegionW : kTileW;
     const int th = strips ? kRegionH : kTileH;
@@ -621,6 +618,16 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
                 C->run_start.as<unsigned>(), C->run_z.as<int>(), C->run_tb.as<int>());
         HIP_TRY(hipGetLastError());
     }
+    if (strips)
+    {
+        TRY(ensure(C->vmeta, T_vis * sizeof(int4)));
+        TRY(ensure(C->vamp, T_vis * sizeof(T4)));
+        hipLaunchKernelGGL(k_gather_visits<T>, dim3(grid_blocks(T_vis, 256, 65535)), dim3(256), 0,
+                C->stream, svals, C->meta.as<int4>(), C->amp.as<T4>(), run_flag, run_excl,
+                C->run_start.as<unsigned>(), C->min_stage_run, kStripMaxW, sc + SC_VISITS,
+                C->vmeta.as<int4>(), C->vamp.as<T4>());
+        HIP_TRY(hipGetLastError());
+    }
 
     // Tiles -> work items.
     TRY(ensure(C->tile_start, num_tiles * sizeof(unsigned)));
@@ -685,11 +692,16 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     ga.min_stage_run = C->min_stage_run;
     ga.meta_bytes = (unsigned)std::min<unsigned long long>(n_total * 16ull, 0xFFFFFFFFull);
     ga.visit_bytes = (unsigned)std::min<unsigned long long>(T_vis * 4ull, 0xFFFFFFFFull);
+    if (strips && (T_vis + 1) * 4ull > 0xFFFFFFFFull)
+        return fail(MI355_ERR_TOO_LARGE, "more than 2^30 tile visits in one call");
     ga.run_bytes = (unsigned)std::min<unsigned long long>((T_vis + 1) * 4ull, 0xFFFFFFFFull);
+    ga.vmeta = C->vmeta.as<int4>();
+    ga.vamp = C->vamp.as<T4>();
+    ga.vrec_bytes = 0;   // (the strip kernel sizes its record windows per item)
     const unsigned padded = (unsigned)((num_items + 7) & ~7ull);
     if (num_items && strips)
     {
-        // One workgroup of 16 waves per work item, in item order.
+        // One workgroup of kStripWaves waves per work item, in item order.
         if constexpr (std::is_same<T, float>::value)
         {
             hipLaunchKernelGGL(k_grid_strips, dim3((unsigned)num_items), dim3(kStripBlock), 0,
