@@ -7,13 +7,22 @@ gridding pipeline over all visibilities: kernel-table expansion, selection,
 bucketing by tile, tile gridding and the ordered merge, accumulating into
 the device grid (the drop-in's semantics).
 
-Prints one JSON line.  `roofline` prices the tile-gridding kernel: its
+Prints one JSON line.  `roofline` prices the gridding kernel: its
 algorithmic FP32 work, 8 flops per complex tap MAC summed over gridded
 visibilities' (2s+1)^2 taps, divided by the kernel's average duration from
-HIP events recorded on the launch stream inside the timed steps.
+HIP events recorded on the launch stream inside the timed steps.  The work
+is FP32 vector FMAs (no matrix instructions: the tap lookup is an indexed
+gather); `peak` is MI355X's FP32 figure, 157.3 TFLOP/s, which needs packed
+FMAs, and `frac_nonpacked` prices against the 78.6 TFLOP/s of one FMA per
+lane per clock.  `traffic` is the kernel's HBM bytes per launch from the
+FETCH_SIZE / WRITE_SIZE counter summary in profiles/, when present.
 `cpu_baseline` times the reference's serial oskar_grid_wproj_f (compiled
 from its sources into oracle/_ref, else the oracle's restatement) on one
-core on a prefix of the same visibilities.
+core on a prefix of the same visibilities.  `dropin` (with --dropin) times
+the host-pointer drop-in mi355_grid_wproj_f, host-device copies included.
+
+--copy-calibration MIB only runs plain device copies of MIB MiB (the
+reference kernel for scaling the HBM byte counters) and exits.
 """
 import argparse
 import ctypes
@@ -27,8 +36,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "gpu-gridding_amd"))
 
-PEAK_FP32_TFLOPS = 157.3          # MI355X vector/matrix FP32, packed FMA
+PEAK_FP32_TFLOPS = 157.3          # MI355X FP32, packed FMA
 PEAK_FP32_NONPACKED_TFLOPS = 78.6  # one FMA per lane per clock
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def log(msg):
@@ -78,6 +88,10 @@ def main():
                          "tiles with tables staged in LDS / read directly")
     ap.add_argument("--visits-per-item", type=int, default=2048)
     ap.add_argument("--min-stage-run", type=int, default=4)
+    ap.add_argument("--dropin", action="store_true",
+                    help="also time the host-pointer drop-in (PCIe included)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run plain device copies of MIB MiB, then exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (the gridder has no multi-GPU "
@@ -89,6 +103,15 @@ def main():
     from mi355_gridding import synth, _lib, torch_ops as ops
 
     torch.cuda.set_device(0)
+    if a.copy_calibration:
+        n = a.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copied %d MiB 5 times" % a.copy_calibration)
+        return
     t0 = time.time()
     case = synth.make(a.config)
     log("generated %s: N=%d G=%d P=%d smax=%d in %.1fs" % (
@@ -167,6 +190,32 @@ def main():
         del g64, gc, g32
         log("accuracy %s" % json.dumps(accuracy))
 
+    dropin = None
+    if a.dropin:
+        from mi355_gridding import wproj
+        hgrid = np.zeros((G, G), np.complex64)
+        times = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            wproj.grid_wproj_f(case["num_w_planes"], case["support"], case["oversample"],
+                               case["conv_size_half"], case["conv_func"], N, case["uu"],
+                               case["vv"], case["ww"], case["vis"], case["weight"],
+                               case["cell_size_rad"], case["w_scale"], G, 0.0, hgrid)
+            times.append(time.perf_counter() - t0)
+        del hgrid
+        dropin = dict(ms_per_call=round(min(times) * 1e3, 3),
+                      value=round(N / min(times) / 1e6, 3), unit="Mvis/s",
+                      note="host arrays; inputs and the touched grid window copied "
+                           "host<->device inside the call (best of 2)")
+        log("drop-in %s" % json.dumps(dropin))
+
+    traffic = None
+    kname = {"strips": "k_grid_strips", "staged": "k_grid_staged", "direct": "k_grid_tiles"}[a.kernel]
+    if os.path.exists(TRAFFIC_FILE):
+        tr = json.load(open(TRAFFIC_FILE)).get(kname)
+        if tr and tr.get("workload") == case["name"]:
+            traffic = tr["bytes_per_launch"]
+
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(case, a.cpu_seconds)
@@ -190,19 +239,19 @@ def main():
                    "oversample": case["oversample"],
                    "accumulation": "compensated" if a.compensated else "plain",
                    "tile_kernel": a.kernel, "visits_per_item": a.visits_per_item},
-        "roofline": {"bound": "fp32-valu", "achieved": round(achieved, 3),
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "frac_nonpacked": round(achieved / PEAK_FP32_NONPACKED_TFLOPS, 4),
-                     "kernel": {"strips": "k_grid_strips", "staged": "k_grid_staged",
-                                "direct": "k_grid_tiles"}[a.kernel],
-                     "kernel_ms": round(grid_ms, 4),
-                     "flops_per_launch": flops, "traffic": None},
+                     "compute": "FP32 vector FMA (compute-bound class; no matrix instructions)",
+                     "kernel": kname, "kernel_ms": round(grid_ms, 4),
+                     "flops_per_launch": flops, "traffic": traffic},
         "phases_ms": phase_ms,
         "work": {"num_taps": st["num_taps"], "num_visits": st["num_visits"],
                  "num_items": st["num_items"], "num_skipped": st["num_skipped"]},
         "accuracy": accuracy,
         "cpu_baseline": cpu,
+        "dropin": dropin,
     }
     print(json.dumps(out), flush=True)
 
