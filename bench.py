@@ -73,6 +73,90 @@ def cpu_baseline(case, seconds):
                        "%s" % (n, case["num_points"], t, platform.processor() or "x86-64"))
 
 
+def bench_batched(a):
+    """Config 3: 64 snapshot grids (4096^2) with log-uniform visibility
+    counts, autotuned, gridded as one captured HIP graph per step; compared
+    with the same pass launched eagerly and with 64 sequential single-grid
+    calls."""
+    import numpy as np
+    import torch
+    from mi355_gridding import synth, _lib, torch_ops as ops
+
+    t0 = time.time()
+    b = synth.make_batched()
+    log("generated %d jobs, %d visibilities in %.1fs" % (
+        len(b["jobs"]), sum(len(j["uu"]) for j in b["jobs"]), time.time() - t0))
+    ks = ops.KernelStack(b["support"], b["oversample"], b["conv_size_half"], b["conv_func"])
+    cat = {k: torch.from_numpy(np.ascontiguousarray(
+               np.concatenate([j[k] for j in b["jobs"]]).reshape(-1))).cuda()
+           for k in ("uu", "vv", "ww", "vis", "weight")}
+    N = cat["uu"].numel()
+    jobs, off = [], 0
+    for j in b["jobs"]:
+        G, n = j["grid_size"], len(j["uu"])
+        jobs.append(dict(vis_offset=off, num_points=n, cell_size_rad=j["cell_size_rad"],
+                         w_scale=j["w_scale"], grid_size=G,
+                         grid=torch.zeros((G, G, 2), dtype=torch.float32, device="cuda"),
+                         norm=torch.zeros(1, dtype=torch.float64, device="cuda"),
+                         num_skipped=torch.zeros(1, dtype=torch.int64, device="cuda")))
+        off += n
+    args = (ks, jobs, cat["uu"], cat["vv"], cat["ww"], cat["vis"], cat["weight"])
+    ctx = ops.context()
+    best = ops.autotune_batched(*args, ctx=ctx)
+    log("autotuned: %.3f ms per pass" % best)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    graph_ms = timed(lambda: ops.grid_batched_graph(*args, ctx=ctx), a.steps, max(a.warmup, 1))
+    st = ctx.stats()                      # counts of the last graph launch
+    eager_ms = timed(lambda: ops.grid_batched(*args, ctx=ctx), max(2, a.steps // 2), 1)
+    singles = []
+    for jb in jobs:
+        o, n = jb["vis_offset"], jb["num_points"]
+        singles.append(tuple(cat[k][o:o + n] for k in ("uu", "vv", "ww")) +
+                       (cat["vis"][2 * o:2 * (o + n)], cat["weight"][o:o + n], jb))
+
+    def sequential():
+        for uu, vv, ww, vis, wt, jb in singles:
+            ops.grid(ks, uu, vv, ww, vis, wt, jb["cell_size_rad"], jb["w_scale"], jb["grid_size"],
+                     jb["grid"], jb["norm"], jb["num_skipped"], ctx=ctx)
+    seq_ms = timed(sequential, 2, 1)
+    flops = 8.0 * st["num_taps"]
+    out = {
+        "metric": "Mvis/s, 64 snapshot grids (4096^2, 339 planes, support 44) in one HIP graph",
+        "value": round(N / (graph_ms * 1e-3) / 1e6, 3),
+        "unit": "Mvis/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(graph_ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded generator, config-3 shapes)",
+        "config": {"workload": "batched-64x4096", "num_jobs": len(jobs), "num_points": N,
+                   "grid_size": 4096, "num_w_planes": b["num_w_planes"],
+                   "max_support": b["max_support"],
+                   "autotuned": {"grid_kernel": ctx.get_option(_lib.OPT_GRID_KERNEL),
+                                 "visits_per_item": ctx.get_option(_lib.OPT_VISITS_PER_ITEM),
+                                 "min_stage_run": ctx.get_option(_lib.OPT_MIN_STAGE_RUN)}},
+        "batched": {"graph_ms": round(graph_ms, 3), "eager_batched_ms": round(eager_ms, 3),
+                    "sequential_64_ms": round(seq_ms, 3),
+                    "graph_vs_sequential": round(seq_ms / graph_ms, 3),
+                    "autotune_best_ms": round(best, 3),
+                    "pipeline_tflops": round(flops / (graph_ms * 1e-3) / 1e12, 3)},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +187,8 @@ def main():
     from mi355_gridding import synth, _lib, torch_ops as ops
 
     torch.cuda.set_device(0)
+    if a.config == 3:
+        return bench_batched(a)
     if a.copy_calibration:
         n = a.copy_calibration * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device="cuda")

@@ -135,6 +135,21 @@ struct mi355_ctx
     bool cache_kernels = false;
     int grid_kernel = 0;            // 0 auto (strips for plain FP32), 1 direct 64x64, 2 staged 64x64
     unsigned min_stage_run = 4;
+    int num_cu = 256;
+
+    // Host-sync-free mode (buffers sized from upper bounds, counts read on
+    // the device) and stream capture in progress; set by the graph entry.
+    bool sync_free = false;
+    bool capturing = false;
+    bool stats_deferred = false;    // stats to be read from the device
+    unsigned long long deferred_points = 0;
+    int deferred_jobs = 0;
+    // captured batched pipeline
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t capture_stream = nullptr;
+    std::vector<unsigned char> graph_key;
+    bool stats_untimed = false;      // last call was a graph launch (no phase events)
 
     // prepared kernel tables
     const void* cache_conv = nullptr;
@@ -177,7 +192,7 @@ mi355_ctx* g_default[64] = {};
 
 void record(mi355_ctx* C, Phase p)
 {
-    if (C->timing) (void)hipEventRecord(C->ev[p], C->stream);
+    if (C->timing && !C->capturing) (void)hipEventRecord(C->ev[p], C->stream);
 }
 
 int init_ctx(mi355_ctx* C, int device)
@@ -192,6 +207,10 @@ int init_ctx(mi355_ctx* C, int device)
         return fail(MI355_ERR_INVALID_ARG, "device %d out of range (have %d)", device, n);
     C->device = device;
     HIP_TRY(hipSetDevice(device));
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+            ncu > 0)
+        C->num_cu = ncu;
     for (int i = 0; i < EV_COUNT; ++i) HIP_TRY(hipEventCreate(&C->ev[i]));
     return MI355_OK;
 }
@@ -258,7 +277,7 @@ int prepare_tables(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
         const typename V2<TI>::t* conv_dev)
 {
     const int prec = (int)(16 * sizeof(TI) + sizeof(TA));
-    if (C->cache_kernels && C->tables_valid && C->cache_conv == conv_dev &&
+    if (!C->sync_free && C->cache_kernels && C->tables_valid && C->cache_conv == conv_dev &&
             C->cache_prec == prec && C->cache_ov == ov && C->cache_csh == csh &&
             C->cache_support.size() == P &&
             std::equal(support, support + P, C->cache_support.begin()))
@@ -304,10 +323,13 @@ int prepare_tables(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     TRY(ensure(C->support_d, P * sizeof(int)));
     TRY(ensure(C->plane_off_d, P * sizeof(unsigned)));
     C->cache_support.assign(support, support + P);
-    HIP_TRY(hipMemcpyAsync(C->support_d.p, C->cache_support.data(), P * sizeof(int),
-            hipMemcpyHostToDevice, C->stream));
-    HIP_TRY(hipMemcpyAsync(C->plane_off_d.p, C->plane_off.data(), P * sizeof(unsigned),
-            hipMemcpyHostToDevice, C->stream));
+    if (!C->capturing)   // (a captured pipeline reuses what the eager run uploaded)
+    {
+        HIP_TRY(hipMemcpyAsync(C->support_d.p, C->cache_support.data(), P * sizeof(int),
+                hipMemcpyHostToDevice, C->stream));
+        HIP_TRY(hipMemcpyAsync(C->plane_off_d.p, C->plane_off.data(), P * sizeof(unsigned),
+                hipMemcpyHostToDevice, C->stream));
+    }
     {
         const unsigned per_plane = (unsigned)((Wmax + 2 * kTablePadRows) * Wmax * noff * noff);
         dim3 g(grid_blocks(per_plane, 256, 1024), (unsigned)std::min<size_t>(P, 65535));
@@ -410,7 +432,9 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     // for; the 64 x 64 kernels take every other case.
     const bool comp = C->compensated;
     const bool strips = std::is_same<T, float>::value && !comp && C->grid_kernel == 0;
-    const bool direct64 = !strips && C->grid_kernel == 1;
+    // (the direct 64 x 64 kernel launches one workgroup per item: not in the
+    // host-sync-free mode, which takes the staged one instead)
+    const bool direct64 = !strips && C->grid_kernel == 1 && !C->sync_free;
     const int tw = strips ? kRegionW : kTileW;
     const int th = strips ? kRegionH : kTileH;
 
@@ -478,10 +502,13 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     TRY(ensure(C->job_vis_off_d, (num_jobs + 1) * sizeof(unsigned long long)));
     TRY(ensure(C->job_out_d, 3 * num_jobs * sizeof(unsigned long long)));
     TRY(ensure(C->job_bbox_d, num_jobs * sizeof(int4)));
-    HIP_TRY(hipMemcpyAsync(C->jobs_d.p, C->h_jobs.data(), C->h_jobs.size(), hipMemcpyHostToDevice, C->stream));
-    HIP_TRY(hipMemcpyAsync(C->job_chunk_off_d.p, C->h_job_chunk_off.data(), (num_jobs + 1) * sizeof(unsigned), hipMemcpyHostToDevice, C->stream));
-    HIP_TRY(hipMemcpyAsync(C->job_tile_base_d.p, C->h_job_tile_base.data(), (num_jobs + 1) * sizeof(unsigned), hipMemcpyHostToDevice, C->stream));
-    HIP_TRY(hipMemcpyAsync(C->job_vis_off_d.p, C->h_job_vis_off.data(), (num_jobs + 1) * sizeof(unsigned long long), hipMemcpyHostToDevice, C->stream));
+    if (!C->capturing)
+    {
+        HIP_TRY(hipMemcpyAsync(C->jobs_d.p, C->h_jobs.data(), C->h_jobs.size(), hipMemcpyHostToDevice, C->stream));
+        HIP_TRY(hipMemcpyAsync(C->job_chunk_off_d.p, C->h_job_chunk_off.data(), (num_jobs + 1) * sizeof(unsigned), hipMemcpyHostToDevice, C->stream));
+        HIP_TRY(hipMemcpyAsync(C->job_tile_base_d.p, C->h_job_tile_base.data(), (num_jobs + 1) * sizeof(unsigned), hipMemcpyHostToDevice, C->stream));
+        HIP_TRY(hipMemcpyAsync(C->job_vis_off_d.p, C->h_job_vis_off.data(), (num_jobs + 1) * sizeof(unsigned long long), hipMemcpyHostToDevice, C->stream));
+    }
 
     // Selection.
     TRY(ensure(C->meta, n_total * sizeof(int4)));
@@ -539,26 +566,46 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     unsigned long long host_sc[SC_COUNT];
     C->h_job_out.resize(3 * num_jobs);
     C->h_job_bbox.resize(num_jobs);
-    HIP_TRY(hipMemcpyAsync(host_sc, sc, sizeof(host_sc), hipMemcpyDeviceToHost, C->stream));
-    HIP_TRY(hipMemcpyAsync(C->h_job_out.data(), C->job_out_d.p, 3 * num_jobs * sizeof(unsigned long long), hipMemcpyDeviceToHost, C->stream));
-    HIP_TRY(hipMemcpyAsync(C->h_job_bbox.data(), C->job_bbox_d.p, num_jobs * sizeof(int4), hipMemcpyDeviceToHost, C->stream));
-    HIP_TRY(hipStreamSynchronize(C->stream));
-    if (host_sc[SC_OVERFLOW]) return fail(MI355_ERR_TOO_LARGE, "more than 2^32-1 tile visits in one call");
-    const unsigned long long T_vis = host_sc[SC_VISITS];
-    unsigned long long num_nonfinite = 0;
+    // T_vis sizes the buffers from here on: the exact count of tile visits,
+    // or (host-sync-free) an upper bound from the largest support, the
+    // kernels reading the exact count on the device.
+    unsigned long long T_vis = 0, num_nonfinite = 0;
     C->stats.num_points = n_total;
-    for (int j = 0; j < num_jobs; ++j)
+    C->stats_deferred = C->sync_free;
+    if (C->sync_free)
     {
-        C->stats.num_skipped += C->h_job_out[3 * j];
-        C->stats.num_taps += C->h_job_out[3 * j + 1];
-        num_nonfinite += C->h_job_out[3 * j + 2];
+        if (before_grid) return fail(MI355_ERR_INVALID_ARG, "host window upload needs host syncs");
+        int smax = 0;
+        for (size_t p = 0; p < P; ++p) smax = std::max(smax, support[p]);
+        const unsigned long long per = (unsigned long long)(2 * smax / tw + 2) * (2 * smax / th + 2);
+        T_vis = std::max(1ull, n_total * per);
+        if (T_vis > 0xFFFFFFFFull)
+            return fail(MI355_ERR_TOO_LARGE, "tile-visit bound exceeds 2^32-1 (split the call)");
+        num_nonfinite = 1;            // (not known: the pass always runs)
+        C->deferred_points = n_total;
+        C->deferred_jobs = num_jobs;
     }
-    C->stats.num_visits = T_vis;
-    if (before_grid) TRY((*before_grid)(C, C->h_job_bbox.data()));
-    if (T_vis == 0)
+    else
     {
-        record(C, EV_BUCKET); record(C, EV_GRID); record(C, EV_MERGE);
-        return grid_nonfinite<T>(C, num_nonfinite, num_jobs, n_total);
+        HIP_TRY(hipMemcpyAsync(host_sc, sc, sizeof(host_sc), hipMemcpyDeviceToHost, C->stream));
+        HIP_TRY(hipMemcpyAsync(C->h_job_out.data(), C->job_out_d.p, 3 * num_jobs * sizeof(unsigned long long), hipMemcpyDeviceToHost, C->stream));
+        HIP_TRY(hipMemcpyAsync(C->h_job_bbox.data(), C->job_bbox_d.p, num_jobs * sizeof(int4), hipMemcpyDeviceToHost, C->stream));
+        HIP_TRY(hipStreamSynchronize(C->stream));
+        if (host_sc[SC_OVERFLOW]) return fail(MI355_ERR_TOO_LARGE, "more than 2^32-1 tile visits in one call");
+        T_vis = host_sc[SC_VISITS];
+        for (int j = 0; j < num_jobs; ++j)
+        {
+            C->stats.num_skipped += C->h_job_out[3 * j];
+            C->stats.num_taps += C->h_job_out[3 * j + 1];
+            num_nonfinite += C->h_job_out[3 * j + 2];
+        }
+        C->stats.num_visits = T_vis;
+        if (before_grid) TRY((*before_grid)(C, C->h_job_bbox.data()));
+        if (T_vis == 0)
+        {
+            record(C, EV_BUCKET); record(C, EV_GRID); record(C, EV_MERGE);
+            return grid_nonfinite<T>(C, num_nonfinite, num_jobs, n_total);
+        }
     }
 
     // Emit (tile | table) keys and sort them.
@@ -652,9 +699,22 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
         TRY(scan_u32(C, C->n_slots.as<unsigned>(), C->slot_off.as<unsigned>(), cnt, num_tiles,
                 sc + SC_SLOTS, reinterpret_cast<unsigned*>(sc + SC_OVERFLOW), 1));
     }
-    HIP_TRY(hipMemcpyAsync(host_sc, sc, sizeof(host_sc), hipMemcpyDeviceToHost, C->stream));
-    HIP_TRY(hipStreamSynchronize(C->stream));
-    const unsigned long long num_items = host_sc[SC_ITEMS], num_slots = host_sc[SC_SLOTS];
+    unsigned long long num_items, num_slots;
+    if (C->sync_free)
+    {
+        // a tile of c visits has ceil(c / V) items, and slots only when that
+        // is 2 or more, i.e. fewer than 2c / V
+        const unsigned long long V = C->visits_per_item;
+        num_items = std::min<unsigned long long>(num_tiles, T_vis) + T_vis / V + 1;
+        num_slots = 2 * T_vis / V + 1;
+    }
+    else
+    {
+        HIP_TRY(hipMemcpyAsync(host_sc, sc, sizeof(host_sc), hipMemcpyDeviceToHost, C->stream));
+        HIP_TRY(hipStreamSynchronize(C->stream));
+        num_items = host_sc[SC_ITEMS];
+        num_slots = host_sc[SC_SLOTS];
+    }
     TRY(ensure(C->item_tile, num_items * sizeof(unsigned)));
     hipLaunchKernelGGL(k_item_map, dim3(tb), dim3(256), 0, C->stream, C->item_off.as<unsigned>(),
             num_tiles, C->item_tile.as<unsigned>());
@@ -699,31 +759,33 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     ga.vamp = C->vamp.as<T4>();
     ga.vrec_bytes = 0;   // (the strip kernel sizes its record windows per item)
     const unsigned padded = (unsigned)((num_items + 7) & ~7ull);
+    // One workgroup per work item, in item order (consecutive items of a
+    // heavy tile land on different XCDs); host-sync-free, a persistent grid
+    // of two workgroups per CU walks the items the device counted.
+    const unsigned wg_items = C->sync_free
+            ? (unsigned)std::min<unsigned long long>(num_items, 2ull * C->num_cu)
+            : (unsigned)num_items;
     if (num_items && strips)
     {
-        // One workgroup of kStripWaves waves per work item, in item order.
         if constexpr (std::is_same<T, float>::value)
         {
-            hipLaunchKernelGGL(k_grid_strips, dim3((unsigned)num_items), dim3(kStripBlock), 0,
-                    C->stream, ga);
+            hipLaunchKernelGGL(k_grid_strips, dim3(wg_items), dim3(kStripBlock), 0, C->stream, ga);
             HIP_TRY(hipGetLastError());
         }
     }
     else if (num_items && !direct64)
     {
-        // One workgroup per work item, in item order (consecutive items of a
-        // heavy tile land on different XCDs).
         if constexpr (std::is_same<T, float>::value)
         {
             if (comp)
-                hipLaunchKernelGGL((k_grid_staged<T, true>), dim3((unsigned)num_items),
+                hipLaunchKernelGGL((k_grid_staged<T, true>), dim3(wg_items),
                         dim3(kStageBlock), 0, C->stream, ga);
             else
-                hipLaunchKernelGGL((k_grid_staged<T, false>), dim3((unsigned)num_items),
+                hipLaunchKernelGGL((k_grid_staged<T, false>), dim3(wg_items),
                         dim3(kStageBlock), 0, C->stream, ga);
         }
         else
-            hipLaunchKernelGGL((k_grid_staged<T, false>), dim3((unsigned)num_items),
+            hipLaunchKernelGGL((k_grid_staged<T, false>), dim3(wg_items),
                     dim3(kStageBlock), 0, C->stream, ga);
         HIP_TRY(hipGetLastError());
     }
@@ -999,6 +1061,35 @@ int degrid_dev(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     return ok();
 }
 
+// Job table of the batched entry points -> per-job host descriptions.
+int batched_jobs(size_t P, const int* support, int ov, int csh, const float* conv,
+        int num_jobs, const mi355_job_f* jobs, const float* uu, const float* vv,
+        const float* ww, const float* vis, const float* weight,
+        std::vector<JobHost<float, float>>& J)
+{
+    TRY(check_kernel_args(P, support, ov, csh, conv));
+    if (num_jobs < 1 || !jobs) return fail(MI355_ERR_INVALID_ARG, "no jobs");
+    J.assign(num_jobs, JobHost<float, float>());
+    unsigned long long n = 0;
+    for (int j = 0; j < num_jobs; ++j)
+    {
+        if (!jobs[j].grid || !jobs[j].norm || !jobs[j].num_skipped)
+            return fail(MI355_ERR_INVALID_ARG, "job %d: NULL output pointer", j);
+        J[j].vis_offset = jobs[j].vis_offset;
+        J[j].num_points = jobs[j].num_points;
+        J[j].cell_size_rad = jobs[j].cell_size_rad;
+        J[j].w_scale = jobs[j].w_scale;
+        J[j].grid_size = jobs[j].grid_size;
+        J[j].grid = reinterpret_cast<float2*>(jobs[j].grid);
+        J[j].norm = jobs[j].norm;
+        J[j].skipped = jobs[j].num_skipped;
+        n += jobs[j].num_points;
+    }
+    if (n && (!uu || !vv || !ww || !vis || !weight))
+        return fail(MI355_ERR_INVALID_ARG, "NULL visibility array");
+    return MI355_OK;
+}
+
 } // namespace
 
 // ===========================================================================
@@ -1061,6 +1152,9 @@ MI355_EXPORT int mi355_ctx_create(int device, mi355_ctx** ctx)
 
 MI355_EXPORT int mi355_ctx_destroy(mi355_ctx* C)
 {
+    if (C && C->graph_exec) (void)hipGraphExecDestroy(C->graph_exec);
+    if (C && C->graph) (void)hipGraphDestroy(C->graph);
+    if (C && C->capture_stream) (void)hipStreamDestroy(C->capture_stream);
     if (!C) return ok();
     (void)hipSetDevice(C->device);
     (void)hipStreamSynchronize(C->stream);
@@ -1112,9 +1206,48 @@ MI355_EXPORT int mi355_ctx_set_option(mi355_ctx* C, int key, long long value)
     return ok();
 }
 
-MI355_EXPORT int mi355_ctx_get_stats(const mi355_ctx* C, mi355_stats* stats)
+MI355_EXPORT int mi355_ctx_get_option(const mi355_ctx* C, int key, long long* value)
 {
-    if (!C || !stats) return fail(MI355_ERR_INVALID_ARG, "NULL argument");
+    if (!C || !value) return fail(MI355_ERR_INVALID_ARG, "NULL argument");
+    switch (key)
+    {
+    case MI355_OPT_VISITS_PER_ITEM: *value = C->visits_per_item; break;
+    case MI355_OPT_COMPENSATED: *value = C->compensated; break;
+    case MI355_OPT_PHASE_TIMING: *value = C->timing; break;
+    case MI355_OPT_CACHE_KERNELS: *value = C->cache_kernels; break;
+    case MI355_OPT_GRID_KERNEL: *value = C->grid_kernel; break;
+    case MI355_OPT_MIN_STAGE_RUN: *value = C->min_stage_run; break;
+    default: return fail(MI355_ERR_INVALID_ARG, "unknown option %d", key);
+    }
+    return ok();
+}
+
+MI355_EXPORT int mi355_ctx_get_stats(const mi355_ctx* Cc, mi355_stats* stats)
+{
+    if (!Cc || !stats) return fail(MI355_ERR_INVALID_ARG, "NULL argument");
+    mi355_ctx* C = const_cast<mi355_ctx*>(Cc);
+    if (C->stats_deferred)
+    {
+        // host-sync-free call: read the counts the device kept
+        HIP_TRY(hipSetDevice(C->device));
+        unsigned long long sc[SC_COUNT];
+        std::vector<unsigned long long> jo(3 * (size_t)C->deferred_jobs);
+        HIP_TRY(hipMemcpyAsync(sc, C->scalars.p, sizeof(sc), hipMemcpyDeviceToHost, C->stream));
+        HIP_TRY(hipMemcpyAsync(jo.data(), C->job_out_d.p, jo.size() * sizeof(unsigned long long),
+                hipMemcpyDeviceToHost, C->stream));
+        HIP_TRY(hipStreamSynchronize(C->stream));
+        C->stats.num_points = C->deferred_points;
+        C->stats.num_visits = sc[SC_VISITS];
+        C->stats.num_items = sc[SC_ITEMS];
+        C->stats.num_skipped = C->stats.num_taps = 0;
+        for (int j = 0; j < C->deferred_jobs; ++j)
+        {
+            C->stats.num_skipped += jo[3 * j];
+            C->stats.num_taps += jo[3 * j + 1];
+        }
+        if (!C->stats_untimed) collect_timing(C);
+        C->stats_deferred = false;
+    }
     *stats = C->stats;
     return ok();
 }
@@ -1199,30 +1332,173 @@ MI355_EXPORT int mi355_grid_wproj_f_batched_dev(mi355_ctx* C, size_t P, const in
         const float* weight)
 {
     if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
-    TRY(check_kernel_args(P, support, ov, csh, conv));
-    if (num_jobs < 1 || !jobs) return fail(MI355_ERR_INVALID_ARG, "no jobs");
-    std::vector<JobHost<float, float>> J(num_jobs);
-    unsigned long long n = 0;
-    for (int j = 0; j < num_jobs; ++j)
-    {
-        if (!jobs[j].grid || !jobs[j].norm || !jobs[j].num_skipped)
-            return fail(MI355_ERR_INVALID_ARG, "job %d: NULL output pointer", j);
-        J[j].vis_offset = jobs[j].vis_offset;
-        J[j].num_points = jobs[j].num_points;
-        J[j].cell_size_rad = jobs[j].cell_size_rad;
-        J[j].w_scale = jobs[j].w_scale;
-        J[j].grid_size = jobs[j].grid_size;
-        J[j].grid = reinterpret_cast<float2*>(jobs[j].grid);
-        J[j].norm = jobs[j].norm;
-        J[j].skipped = jobs[j].num_skipped;
-        n += jobs[j].num_points;
-    }
-    if (n && (!uu || !vv || !ww || !vis || !weight))
-        return fail(MI355_ERR_INVALID_ARG, "NULL visibility array");
+    std::vector<JobHost<float, float>> J;
+    TRY(batched_jobs(P, support, ov, csh, conv, num_jobs, jobs, uu, vv, ww, vis, weight, J));
     HIP_TRY(hipSetDevice(C->device));
+    C->stats_untimed = false;
     TRY((run_pipeline<float, float>(C, P, support, ov, csh, reinterpret_cast<const float2*>(conv),
             num_jobs, J.data(), uu, vv, ww, vis, weight, nullptr)));
     collect_timing(C);
+    return ok();
+}
+
+MI355_EXPORT int mi355_grid_wproj_f_batched_graph_dev(mi355_ctx* C, size_t P,
+        const int* support, int ov, int csh, const float* conv, int num_jobs,
+        const mi355_job_f* jobs, const float* uu, const float* vv, const float* ww,
+        const float* vis, const float* weight)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    std::vector<JobHost<float, float>> J;
+    TRY(batched_jobs(P, support, ov, csh, conv, num_jobs, jobs, uu, vv, ww, vis, weight, J));
+    HIP_TRY(hipSetDevice(C->device));
+    // What the captured launch depends on: shapes, pointers, job table,
+    // options.  The input values themselves may change between launches.
+    std::vector<unsigned char> key;
+    auto put = [&](const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        key.insert(key.end(), b, b + n);
+    };
+    const void* ptrs[6] = {conv, uu, vv, ww, vis, weight};
+    const long long opts[5] = {(long long)C->visits_per_item, (long long)C->min_stage_run,
+                               C->grid_kernel, C->compensated, (long long)(intptr_t)C->stream};
+    put(&P, sizeof(P));
+    put(support, P * sizeof(int));
+    put(&ov, sizeof(ov));
+    put(&csh, sizeof(csh));
+    put(&num_jobs, sizeof(num_jobs));
+    put(jobs, (size_t)num_jobs * sizeof(mi355_job_f));
+    put(ptrs, sizeof(ptrs));
+    put(opts, sizeof(opts));
+    if (C->graph_exec && key == C->graph_key)
+    {
+        HIP_TRY(hipGraphLaunch(C->graph_exec, C->stream));
+        C->stats = mi355_stats();
+        C->stats_deferred = true;
+        C->stats_untimed = true;
+        return ok();
+    }
+    // Build: one eager host-sync-free run (which sizes every buffer and
+    // produces this call's result), then the same sequence captured on the
+    // library's own stream for the following calls.
+    if (C->graph_exec) { (void)hipGraphExecDestroy(C->graph_exec); C->graph_exec = nullptr; }
+    if (C->graph) { (void)hipGraphDestroy(C->graph); C->graph = nullptr; }
+    C->graph_key.clear();
+    C->stats_untimed = false;
+    C->sync_free = true;
+    int st = run_pipeline<float, float>(C, P, support, ov, csh,
+            reinterpret_cast<const float2*>(conv), num_jobs, J.data(), uu, vv, ww, vis, weight,
+            nullptr);
+    const mi355_stats eager_stats = C->stats;
+    const bool eager_deferred = C->stats_deferred;
+    if (st == MI355_OK)
+    {
+        if (!C->capture_stream &&
+                hipStreamCreateWithFlags(&C->capture_stream, hipStreamNonBlocking) != hipSuccess)
+            st = fail(MI355_ERR_HIP, "capture stream");
+        // the capture stream must see the eager run's uploads and buffers
+        else if (hipStreamSynchronize(C->stream) != hipSuccess)
+            st = fail(MI355_ERR_HIP, "stream synchronize");
+    }
+    if (st == MI355_OK)
+    {
+        hipStream_t user = C->stream;
+        C->stream = C->capture_stream;
+        C->capturing = true;
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(C->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            st = fail(MI355_ERR_HIP, "begin capture");
+        else
+        {
+            st = run_pipeline<float, float>(C, P, support, ov, csh,
+                    reinterpret_cast<const float2*>(conv), num_jobs, J.data(), uu, vv, ww, vis,
+                    weight, nullptr);
+            const hipError_t e = hipStreamEndCapture(C->stream, &g);
+            if (st == MI355_OK && (e != hipSuccess || !g))
+                st = fail(MI355_ERR_HIP, "end capture: %s", hipGetErrorString(e));
+        }
+        C->capturing = false;
+        C->stream = user;
+        if (st == MI355_OK)
+        {
+            if (hipGraphInstantiate(&C->graph_exec, g, nullptr, nullptr, 0) != hipSuccess)
+            {
+                C->graph_exec = nullptr;
+                st = fail(MI355_ERR_HIP, "graph instantiate");
+            }
+            else
+            {
+                C->graph = g;
+                C->graph_key = key;
+            }
+        }
+        if (st != MI355_OK && g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+    }
+    C->sync_free = false;
+    C->stats = eager_stats;
+    C->stats_deferred = eager_deferred;
+    return st == MI355_OK ? ok() : st;
+}
+
+MI355_EXPORT int mi355_autotune_f_batched_dev(mi355_ctx* C, size_t P, const int* support,
+        int ov, int csh, const float* conv, int num_jobs, const mi355_job_f* jobs,
+        const float* uu, const float* vv, const float* ww, const float* vis,
+        const float* weight, double* best_ms)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    std::vector<JobHost<float, float>> J;
+    TRY(batched_jobs(P, support, ov, csh, conv, num_jobs, jobs, uu, vv, ww, vis, weight, J));
+    HIP_TRY(hipSetDevice(C->device));
+    // Trial outputs: one scratch grid (the largest), norm and count per
+    // job, so the caller's grids are not touched.
+    int gmax = 0;
+    for (int j = 0; j < num_jobs; ++j) gmax = std::max(gmax, J[j].grid_size);
+    DevBuf tgrid, tnorm, tskip;
+    TRY(ensure(tgrid, (size_t)gmax * gmax * sizeof(float2)));
+    TRY(ensure(tnorm, (size_t)num_jobs * sizeof(double)));
+    TRY(ensure(tskip, (size_t)num_jobs * sizeof(unsigned long long)));
+    for (int j = 0; j < num_jobs; ++j)
+    {
+        J[j].grid = tgrid.as<float2>();         // (results are discarded)
+        J[j].norm = tnorm.as<double>() + j;
+        J[j].skipped = tskip.as<unsigned long long>() + j;
+    }
+    const bool timing0 = C->timing;
+    const unsigned vpi0 = C->visits_per_item, msr0 = C->min_stage_run;
+    const int gk0 = C->grid_kernel;
+    C->timing = true;
+    struct Cand { int kernel; unsigned vpi, msr; };
+    std::vector<Cand> cands;
+    for (int kern : {0, 2})
+        for (unsigned vpi : {1024u, 2048u, 4096u, 8192u})
+            for (unsigned msr : {2u, 4u, 8u})
+                if (kern == 0 || msr == 4u) cands.push_back({kern, vpi, msr});
+    double best = 1e300;
+    Cand bc = {gk0, vpi0, msr0};
+    int st = MI355_OK;
+    for (const Cand& c : cands)
+    {
+        C->grid_kernel = c.kernel;
+        C->visits_per_item = c.vpi;
+        C->min_stage_run = c.msr;
+        double t = 0;
+        for (int rep = 0; rep < 2 && st == MI355_OK; ++rep)   // first run warms up
+        {
+            st = run_pipeline<float, float>(C, P, support, ov, csh,
+                    reinterpret_cast<const float2*>(conv), num_jobs, J.data(), uu, vv, ww, vis,
+                    weight, nullptr);
+            if (st == MI355_OK) { collect_timing(C); t = C->stats.ms_total; }
+        }
+        if (st != MI355_OK) break;
+        if (t < best) { best = t; bc = c; }
+    }
+    C->timing = timing0;
+    C->grid_kernel = st == MI355_OK ? bc.kernel : gk0;
+    C->visits_per_item = st == MI355_OK ? bc.vpi : vpi0;
+    C->min_stage_run = st == MI355_OK ? bc.msr : msr0;
+    release(tgrid); release(tnorm); release(tskip);
+    if (st != MI355_OK) return st;
+    if (best_ms) *best_ms = best;
     return ok();
 }
 

@@ -88,6 +88,7 @@ SIGNATURES = {
     "mi355_default_ctx": (I, [ctypes.POINTER(P)]),
     "mi355_ctx_set_stream": (I, [P, P]),
     "mi355_ctx_set_option": (I, [P, I, ctypes.c_longlong]),
+    "mi355_ctx_get_option": (I, [P, I, ctypes.POINTER(ctypes.c_longlong)]),
     "mi355_ctx_get_stats": (I, [P, ctypes.POINTER(Stats)]),
     "mi355_grid_wproj_f_dev": (I, [P, SZ, P, I, I, P, SZ, P, P, P, P, P, F, F, I, P, P, P]),
     "mi355_grid_wproj_d_dev": (I, [P, SZ, P, I, I, P, SZ, P, P, P, P, P, D, D, I, P, P, P]),
@@ -97,6 +98,10 @@ SIGNATURES = {
     "mi355_degrid_wproj_f_dev": (I, [P, SZ, P, I, I, P, SZ, P, P, P, P, F, F, I, P, I, P]),
     "mi355_degrid_wproj_d_dev": (I, [P, SZ, P, I, I, P, SZ, P, P, P, P, D, D, I, P, I, P]),
     "mi355_grid_wproj_f_batched_dev": (I, [P, SZ, P, I, I, P, I, ctypes.POINTER(JobF), P, P, P, P, P]),
+    "mi355_grid_wproj_f_batched_graph_dev": (I, [P, SZ, P, I, I, P, I, ctypes.POINTER(JobF),
+                                                 P, P, P, P, P]),
+    "mi355_autotune_f_batched_dev": (I, [P, SZ, P, I, I, P, I, ctypes.POINTER(JobF), P, P, P, P, P,
+                                         ctypes.POINTER(D)]),
 }
 
 _lib = None
@@ -165,6 +170,11 @@ class Context:
 
     def set_option(self, key, value):
         check(self._lib.mi355_ctx_set_option(self.handle, key, int(value)))
+
+    def get_option(self, key):
+        v = ctypes.c_longlong(0)
+        check(self._lib.mi355_ctx_get_option(self.handle, key, ctypes.byref(v)))
+        return v.value
 
     def stats(self):
         s = Stats()

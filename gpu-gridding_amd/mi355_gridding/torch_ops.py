@@ -224,14 +224,7 @@ def grid_autograd(vis, weight, ks, uu, vv, ww, cell_size_rad, w_scale, grid_size
     return GridWProj.apply(vis, weight, ks, uu, vv, ww, cell_size_rad, w_scale, grid_size)
 
 
-def grid_batched(ks, jobs, uu, vv, ww, vis, weight, ctx=None):
-    """One pipeline pass over many jobs (mi355_grid_wproj_f_batched_dev).
-
-    jobs: list of dicts with vis_offset, num_points, cell_size_rad, w_scale,
-    grid_size, grid (device tensor), norm (float64 [1]), num_skipped (int64 [1]).
-    uu..weight hold all jobs' visibilities back to back.
-    """
-    ctx = ctx or context()
+def _job_table(ks, jobs, uu, vv, ww, vis, weight):
     if ks.dtype != torch.float32:
         raise ValueError("batched gridding is FP32")
     arr = (_lib.JobF * len(jobs))()
@@ -247,7 +240,41 @@ def grid_batched(ks, jobs, uu, vv, ww, vis, weight, ctx=None):
     for t, name, k in ((uu, "uu", n), (vv, "vv", n), (ww, "ww", n), (weight, "weight", n),
                        (vis, "vis", 2 * n)):
         _check_dev(t, torch.float32, k, name)
-    lib = _lib.load()
-    _lib.check(lib.mi355_grid_wproj_f_batched_dev(
-        ctx.handle, ks.num_w_planes, ks.support_ptr(), ks.oversample, ks.conv_size_half,
-        _dp(ks.conv), len(jobs), arr, _dp(uu), _dp(vv), _dp(ww), _dp(vis), _dp(weight)))
+    return arr
+
+
+def _batched_call(fn, ks, jobs, uu, vv, ww, vis, weight, ctx, *extra):
+    ctx = ctx or context()
+    arr = _job_table(ks, jobs, uu, vv, ww, vis, weight)
+    _lib.check(fn(ctx.handle, ks.num_w_planes, ks.support_ptr(), ks.oversample,
+                  ks.conv_size_half, _dp(ks.conv), len(jobs), arr, _dp(uu), _dp(vv), _dp(ww),
+                  _dp(vis), _dp(weight), *extra))
+
+
+def grid_batched(ks, jobs, uu, vv, ww, vis, weight, ctx=None):
+    """One pipeline pass over many jobs (mi355_grid_wproj_f_batched_dev).
+
+    jobs: list of dicts with vis_offset, num_points, cell_size_rad, w_scale,
+    grid_size, grid (device tensor), norm (float64 [1]), num_skipped (int64 [1]).
+    uu..weight hold all jobs' visibilities back to back.
+    """
+    _batched_call(_lib.load().mi355_grid_wproj_f_batched_dev, ks, jobs, uu, vv, ww, vis,
+                  weight, ctx)
+
+
+def grid_batched_graph(ks, jobs, uu, vv, ww, vis, weight, ctx=None):
+    """The batched pass as one HIP graph (mi355_grid_wproj_f_batched_graph_dev):
+    captured on the first call with these jobs, pointers and options, replayed
+    on the context's stream afterwards."""
+    _batched_call(_lib.load().mi355_grid_wproj_f_batched_graph_dev, ks, jobs, uu, vv, ww, vis,
+                  weight, ctx)
+
+
+def autotune_batched(ks, jobs, uu, vv, ww, vis, weight, ctx=None):
+    """Pick the gridding kernel, work-item size and shortest staged run for
+    this workload (mi355_autotune_f_batched_dev); the context keeps them.
+    Returns the best time per pass in ms."""
+    best = ctypes.c_double(0.0)
+    _batched_call(_lib.load().mi355_autotune_f_batched_dev, ks, jobs, uu, vv, ww, vis, weight,
+                  ctx, ctypes.byref(best))
+    return best.value

@@ -102,6 +102,7 @@ MI355_EXPORT int mi355_ctx_set_stream(mi355_ctx* ctx, void* hip_stream);
                                         2: 64x64 tiles, tables staged in LDS */
 #define MI355_OPT_MIN_STAGE_RUN   6  /* shortest same-table run staged in LDS (default 4) */
 MI355_EXPORT int mi355_ctx_set_option(mi355_ctx* ctx, int key, long long value);
+MI355_EXPORT int mi355_ctx_get_option(const mi355_ctx* ctx, int key, long long* value);
 
 /* Figures of the last gridding call on this context. */
 typedef struct
@@ -218,6 +219,34 @@ MI355_EXPORT int mi355_grid_wproj_f_batched_dev(mi355_ctx* ctx,
         int conv_size_half, const float* conv_func, int num_jobs,
         const mi355_job_f* jobs, const float* uu, const float* vv,
         const float* ww, const float* vis, const float* weight);
+
+/* The same batched pass as one HIP graph launched on the context's stream.
+ * The first call with a given job table, set of pointers and options runs
+ * the pipeline without host synchronisation (buffers sized from upper
+ * bounds, counts kept on the device), then captures that sequence; later
+ * calls with the same table, pointers and options replay the graph, reading
+ * whatever the input arrays hold at the time.  Results arrive through the
+ * job table's device pointers when the stream reaches them;
+ * mi355_ctx_get_stats synchronises the stream and reads the counts back.
+ * No reference counterpart (the reference grids one block per call). */
+MI355_EXPORT int mi355_grid_wproj_f_batched_graph_dev(mi355_ctx* ctx,
+        size_t num_w_planes, const int* support, int oversample,
+        int conv_size_half, const float* conv_func, int num_jobs,
+        const mi355_job_f* jobs, const float* uu, const float* vv,
+        const float* ww, const float* vis, const float* weight);
+
+/* Autotuner for a batched workload: times the pipeline (into scratch
+ * outputs; the job table's grids are not touched) for each gridding kernel
+ * / tile geometry (16-column strips on 192x128 regions, or 64x64 tiles),
+ * work-item size and shortest staged run, and leaves the fastest settings
+ * in the context's options (MI355_OPT_GRID_KERNEL, _VISITS_PER_ITEM,
+ * _MIN_STAGE_RUN).  *best_ms receives that setting's time per pass.  No
+ * reference counterpart. */
+MI355_EXPORT int mi355_autotune_f_batched_dev(mi355_ctx* ctx,
+        size_t num_w_planes, const int* support, int oversample,
+        int conv_size_half, const float* conv_func, int num_jobs,
+        const mi355_job_f* jobs, const float* uu, const float* vv,
+        const float* ww, const float* vis, const float* weight, double* best_ms);
 
 #ifdef __cplusplus
 }

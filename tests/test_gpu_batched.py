@@ -43,3 +43,84 @@ def test_batched_equals_sequential(gpu):
         assert same_bits(g.cpu().numpy(), job["grid"].cpu().numpy())
         assert n.item() == job["norm"].item()
         assert s.item() == job["num_skipped"].item()
+
+
+def _batched_case(torch, ops, num_jobs=5, seed=3):
+    b = synth.make_batched(num_jobs=num_jobs, grid_size=1024, num_w_planes=64, max_support=20,
+                           n_min=2_000, n_max=30_000, seed=seed)
+    b["jobs"][1]["grid_size"] = 2048
+    ks = ops.KernelStack(b["support"], b["oversample"], b["conv_size_half"], b["conv_func"])
+    cat = {k: torch.from_numpy(np.concatenate([j[k] for j in b["jobs"]])).cuda()
+           for k in ("uu", "vv", "ww", "vis", "weight")}
+    jobs, off = [], 0
+    for j in b["jobs"]:
+        G, n = j["grid_size"], len(j["uu"])
+        jobs.append(dict(vis_offset=off, num_points=n, cell_size_rad=j["cell_size_rad"],
+                         w_scale=j["w_scale"], grid_size=G,
+                         grid=torch.zeros((G, G, 2), dtype=torch.float32, device="cuda"),
+                         norm=torch.zeros(1, dtype=torch.float64, device="cuda"),
+                         num_skipped=torch.zeros(1, dtype=torch.int64, device="cuda")))
+        off += n
+    return ks, cat, jobs
+
+
+def _zero(jobs):
+    for job in jobs:
+        job["grid"].zero_()
+        job["norm"].zero_()
+        job["num_skipped"].fill_(-1)
+
+
+def _snap(jobs):
+    return [(job["grid"].cpu().numpy().copy(), job["norm"].item(), job["num_skipped"].item())
+            for job in jobs]
+
+
+def test_graph_replays_equal_eager(gpu):
+    """Graph bar: every replay, with eager work and host synchronisations in
+    between, equals the eager (non-graph) batched pass bit for bit; so does
+    a replay after the inputs' values change in place."""
+    torch, ops = gpu
+    ks, cat, jobs = _batched_case(torch, ops)
+    args = (ks, jobs, cat["uu"], cat["vv"], cat["ww"], cat["vis"], cat["weight"])
+    ctx = ops.context()
+    _zero(jobs)
+    ops.grid_batched(*args, ctx=ctx)
+    torch.cuda.synchronize()
+    ref = _snap(jobs)
+    for rep in range(4):              # first call builds, the others replay
+        _zero(jobs)
+        ops.grid_batched_graph(*args, ctx=ctx)
+        torch.cuda.synchronize()
+        got = _snap(jobs)
+        for (g0, n0, s0), (g1, n1, s1) in zip(ref, got):
+            assert same_bits(g0, g1) and n0 == n1 and s0 == s1, rep
+        # unrelated eager work between replays
+        x = torch.randn(1 << 20, device="cuda")
+        _ = (x * 2).sum().item()
+    st = ctx.stats()
+    assert st["num_points"] == cat["uu"].numel() and st["num_visits"] > 0
+    # new values in the same arrays: the replay grids them
+    cat["vis"].mul_(2)
+    _zero(jobs)
+    ops.grid_batched_graph(*args, ctx=ctx)
+    torch.cuda.synchronize()
+    for (g0, n0, s0), job in zip(ref, jobs):
+        assert same_bits(g0 * 2, job["grid"].cpu().numpy()) and s0 == job["num_skipped"].item()
+
+
+def test_autotune_keeps_results_and_sets_options(gpu):
+    torch, ops = gpu
+    ks, cat, jobs = _batched_case(torch, ops, num_jobs=3, seed=4)
+    args = (ks, jobs, cat["uu"], cat["vv"], cat["ww"], cat["vis"], cat["weight"])
+    ctx = ops.context()
+    _zero(jobs)
+    best = ops.autotune_batched(*args, ctx=ctx)
+    assert best > 0
+    # the caller's outputs were not touched by the trials
+    for job in jobs:
+        assert job["grid"].abs().max().item() == 0 and job["num_skipped"].item() == -1
+    ops.grid_batched(*args, ctx=ctx)
+    torch.cuda.synchronize()
+    for job in jobs:
+        assert job["num_skipped"].item() >= 0
