@@ -134,7 +134,7 @@ struct mi355_ctx
     bool timing = false;
     bool cache_kernels = false;
     int grid_kernel = 0;            // 0 auto (strips for plain FP32), 1 direct 64x64, 2 staged 64x64
-    unsigned min_stage_run = 4;
+    unsigned min_stage_run = 6;
     int num_cu = 256;
 
     // Host-sync-free mode (buffers sized from upper bounds, counts read on

@@ -100,7 +100,7 @@ MI355_EXPORT int mi355_ctx_set_stream(mi355_ctx* ctx, void* hip_stream);
                                         256x128 regions for plain FP32, else 2;
                                         1: 64x64 tiles, tables read directly;
                                         2: 64x64 tiles, tables staged in LDS */
-#define MI355_OPT_MIN_STAGE_RUN   6  /* shortest same-table run staged in LDS (default 4) */
+#define MI355_OPT_MIN_STAGE_RUN   6  /* shortest same-table run staged in LDS (default 6) */
 MI355_EXPORT int mi355_ctx_set_option(mi355_ctx* ctx, int key, long long value);
 MI355_EXPORT int mi355_ctx_get_option(const mi355_ctx* ctx, int key, long long* value);
 

@@ -203,7 +203,7 @@ def test_kernel_variants_against_fp64_oracle(gpu, kernel, min_run):
         g2, _, _ = run_grid(torch, ops, c, ctx=ctx, visits_per_item=53)
     finally:
         ctx.set_option(_lib.OPT_GRID_KERNEL, 0)
-        ctx.set_option(_lib.OPT_MIN_STAGE_RUN, 4)
+        ctx.set_option(_lib.OPT_MIN_STAGE_RUN, 6)
     assert s == ts and abs(n - tn) <= 1e-12 * abs(tn)
     emax, el2 = errors(g, t)
     assert emax <= 1e-5 and el2 <= 1e-5, (kernel, min_run, emax, el2)
