@@ -7,6 +7,7 @@
 #include "gg_kernels.hpp"
 #include "gg_grid_staged.hpp"
 #include "gg_grid_strips.hpp"
+#include "gg_weights.hpp"
 #include "mi355_gridder.h"
 
 #include <algorithm>
@@ -173,6 +174,7 @@ struct mi355_ctx
     DevBuf jobs_d, job_chunk_off_d, job_vis_off_d, job_tile_base_d, job_out_d, job_bbox_d;
     // drop-in staging
     DevBuf in_conv, in_uu, in_vv, in_ww, in_vis, in_weight, in_grid, in_norm, in_skipped;
+    DevBuf in_weight_out;
 
     // host mirrors kept alive across asynchronous uploads
     std::vector<unsigned char> h_jobs;
@@ -370,8 +372,42 @@ int scan_u32(mi355_ctx* C, const unsigned* in, unsigned* out, DevCount cnt,
     return MI355_OK;
 }
 
-// Device scalars (u64 slots in C->scalars).
-enum Scalar { SC_VISITS, SC_OVERFLOW, SC_ITEMS, SC_SLOTS, SC_SCRATCH0, SC_RUNS = 6, SC_COUNT = 8 };
+// Device scalars (u64 slots in C->scalars).  The weighting entry points use
+// their own slots (count and bounding box), so that they leave the counts of
+// a gridding call, read later by mi355_ctx_get_stats, in place.
+enum Scalar { SC_VISITS, SC_OVERFLOW, SC_ITEMS, SC_SLOTS, SC_SCRATCH0, SC_RUNS = 6,
+              SC_W_COUNT = 8, SC_W_BOX = 10, SC_COUNT = 12 };
+
+// Stable LSD radix sort, 8 bits a pass, of the pairs (C->keys[0], C->vals[0])
+// on the low key_bits bits of the keys: *d_n pairs, n_bound an upper bound of
+// *d_n known on the host.  *cur receives the index of the ping-pong buffers
+// holding the result.
+int radix_sort(mi355_ctx* C, unsigned long long n_bound, const unsigned long long* d_n,
+        int key_bits, int* cur_out)
+{
+    unsigned long long* sc = C->scalars.as<unsigned long long>();
+    const unsigned long long nchunk = (n_bound + kSortChunk - 1) / kSortChunk;
+    TRY(ensure(C->hist, 256 * nchunk * sizeof(unsigned)));
+    TRY(ensure(C->hist_off, (256 * nchunk + 1) * sizeof(unsigned)));
+    int cur = 0;
+    for (int shift = 0; shift < key_bits; shift += 8)
+    {
+        const unsigned blocks = grid_blocks(nchunk, 1, 16384);
+        hipLaunchKernelGGL(k_sort_hist, dim3(blocks), dim3(kSortThreads), 0, C->stream,
+                C->keys[cur].as<unsigned>(), d_n, shift, C->hist.as<unsigned>());
+        DevCount hc{d_n, (unsigned long long)kSortChunk, 256, 0};
+        TRY(scan_u32(C, C->hist.as<unsigned>(), C->hist_off.as<unsigned>(), hc, 256 * nchunk,
+                sc + SC_SCRATCH0, reinterpret_cast<unsigned*>(sc + SC_SCRATCH0 + 1), 0));
+        hipLaunchKernelGGL(k_sort_scatter, dim3(blocks), dim3(kSortThreads), 0, C->stream,
+                C->keys[cur].as<unsigned>(), C->vals[cur].as<unsigned>(), d_n, shift,
+                C->hist_off.as<unsigned>(), C->keys[cur ^ 1].as<unsigned>(),
+                C->vals[cur ^ 1].as<unsigned>());
+        HIP_TRY(hipGetLastError());
+        cur ^= 1;
+    }
+    *cur_out = cur;
+    return MI355_OK;
+}
 
 // ---------------------------------------------------------------------------
 // One pass of the pipeline over num_jobs jobs.
@@ -621,24 +657,8 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
             C->vals[0].as<unsigned>());
     HIP_TRY(hipGetLastError());
     const int key_bits = tile_bits + key_table_bits;
-    const unsigned long long nchunk = (T_vis + kSortChunk - 1) / kSortChunk;
-    TRY(ensure(C->hist, 256 * nchunk * sizeof(unsigned)));
-    TRY(ensure(C->hist_off, (256 * nchunk + 1) * sizeof(unsigned)));
     int cur = 0;
-    for (int shift = 0; shift < key_bits; shift += 8)
-    {
-        const unsigned blocks = grid_blocks(nchunk, 1, 16384);
-        hipLaunchKernelGGL(k_sort_hist, dim3(blocks), dim3(kSortThreads), 0, C->stream,
-                C->keys[cur].as<unsigned>(), sc + SC_VISITS, shift, C->hist.as<unsigned>());
-        DevCount hc{sc + SC_VISITS, (unsigned long long)kSortChunk, 256, 0};
-        TRY(scan_u32(C, C->hist.as<unsigned>(), C->hist_off.as<unsigned>(), hc, 256 * nchunk,
-                sc + SC_SCRATCH0, reinterpret_cast<unsigned*>(sc + SC_SCRATCH0 + 1), 0));
-        hipLaunchKernelGGL(k_sort_scatter, dim3(blocks), dim3(kSortThreads), 0, C->stream,
-                C->keys[cur].as<unsigned>(), C->vals[cur].as<unsigned>(), sc + SC_VISITS, shift,
-                C->hist_off.as<unsigned>(), C->keys[cur ^ 1].as<unsigned>(), C->vals[cur ^ 1].as<unsigned>());
-        HIP_TRY(hipGetLastError());
-        cur ^= 1;
-    }
+    TRY(radix_sort(C, T_vis, sc + SC_VISITS, key_bits, &cur));
     const unsigned* skeys = C->keys[cur].as<unsigned>();
     const unsigned* svals = C->vals[cur].as<unsigned>();
 
@@ -1090,6 +1110,210 @@ int batched_jobs(size_t P, const int* support, int ov, int csh, const float* con
     return MI355_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Uniform-weighting pre-pass (oskar_grid_weights.c)
+// ---------------------------------------------------------------------------
+int check_weights_args(size_t N, int G)
+{
+    if (G < 1) return fail(MI355_ERR_INVALID_ARG, "grid_size must be >= 1");
+    // cell indices v * G + u and the off-grid key G^2 fit 32 bits
+    if (G > 65535) return fail(MI355_ERR_TOO_LARGE, "grid_size > 65535 for the weighting pre-pass");
+    if (N > 0xFFFFFFFFull) return fail(MI355_ERR_TOO_LARGE, "more than 2^32-1 points in one call");
+    return MI355_OK;
+}
+
+// Cell keys of the N points, stably sorted (point order kept within a cell);
+// counts points off the grid into *skipped and the cells' bounding box into
+// box (when given).  Expects k_weight_init to have set SC_W_COUNT to N.
+template <typename T>
+int weights_sorted_cells(mi355_ctx* C, size_t N, const T* uu, const T* vv, T gs, int G,
+        unsigned long long* skipped, int* box, int* cur)
+{
+    for (int b = 0; b < 2; ++b)
+    {
+        TRY(ensure(C->keys[b], N * sizeof(unsigned)));
+        TRY(ensure(C->vals[b], N * sizeof(unsigned)));
+    }
+    const unsigned long long cells = (unsigned long long)G * G;
+    hipLaunchKernelGGL(k_weight_cells<T>, dim3(grid_blocks(N, 256, 16384)), dim3(256), 0,
+            C->stream, uu, vv, (unsigned long long)N, gs, G, (unsigned)cells,
+            C->keys[0].as<unsigned>(), C->vals[0].as<unsigned>(), skipped, box);
+    HIP_TRY(hipGetLastError());
+    return radix_sort(C, N, C->scalars.as<unsigned long long>() + SC_W_COUNT,
+            bits_for(cells + 1), cur);
+}
+
+template <typename T>
+int weights_write_dev(mi355_ctx* C, size_t N, const T* uu, const T* vv, const T* weight,
+        T cell, int G, unsigned long long* skipped, T* grid)
+{
+    TRY(check_weights_args(N, G));
+    if (!skipped || !grid) return fail(MI355_ERR_INVALID_ARG, "NULL output pointer");
+    if (N && (!uu || !vv || !weight)) return fail(MI355_ERR_INVALID_ARG, "NULL input array");
+    HIP_TRY(hipSetDevice(C->device));
+    TRY(ensure(C->scalars, SC_COUNT * sizeof(unsigned long long)));
+    unsigned long long* sc = C->scalars.as<unsigned long long>();
+    hipLaunchKernelGGL(k_weight_init, dim3(1), dim3(64), 0, C->stream, sc + SC_W_COUNT,
+            (unsigned long long)N, skipped, (int*)nullptr);
+    HIP_TRY(hipGetLastError());
+    if (N == 0) return ok();
+    const T gs = (T)G * cell;     // oskar_grid_weights.c:113 / :44
+    int cur = 0;
+    TRY(weights_sorted_cells<T>(C, N, uu, vv, gs, G, skipped, nullptr, &cur));
+    hipLaunchKernelGGL(k_weight_sum<T>, dim3(grid_blocks(N, 256, 65535)), dim3(256), 0,
+            C->stream, C->keys[cur].as<unsigned>(), C->vals[cur].as<unsigned>(), weight,
+            (unsigned long long)N, (unsigned)((unsigned long long)G * G), G, grid, 0, 0,
+            (unsigned long long)G);
+    HIP_TRY(hipGetLastError());
+    return ok();
+}
+
+template <typename T>
+int weights_read_dev(mi355_ctx* C, size_t N, const T* uu, const T* vv, const T* weight_in,
+        T* weight_out, T cell, int G, unsigned long long* skipped, const T* grid)
+{
+    TRY(check_weights_args(N, G));
+    if (!skipped || !grid) return fail(MI355_ERR_INVALID_ARG, "NULL pointer argument");
+    if (N && (!uu || !vv || !weight_in || !weight_out))
+        return fail(MI355_ERR_INVALID_ARG, "NULL point array");
+    HIP_TRY(hipSetDevice(C->device));
+    hipLaunchKernelGGL(k_weight_init, dim3(1), dim3(64), 0, C->stream,
+            (unsigned long long*)nullptr, 0ull, skipped, (int*)nullptr);
+    if (N)
+    {
+        const T gs = (T)G * cell;     // oskar_grid_weights.c:146 / :77
+        hipLaunchKernelGGL(k_weight_read<T>, dim3(grid_blocks(N, 256, 16384)), dim3(256), 0,
+                C->stream, uu, vv, weight_in, weight_out, (unsigned long long)N, gs, G, grid, 0, 0,
+                (unsigned long long)G, skipped);
+    }
+    HIP_TRY(hipGetLastError());
+    return ok();
+}
+
+// Host-pointer drop-ins: the points go to the device, and only the window of
+// the weights grid that the points' cells span travels each way.
+template <typename T>
+int upload_weight_window(mi355_ctx* C, const T* grid, int G, const int* box)
+{
+    const size_t ww = (size_t)(box[2] - box[0] + 1), wh = (size_t)(box[3] - box[1] + 1);
+    TRY(ensure(C->in_grid, ww * wh * sizeof(T)));
+    HIP_TRY(hipMemcpy2DAsync(C->in_grid.p, ww * sizeof(T), grid + (size_t)box[1] * G + box[0],
+            (size_t)G * sizeof(T), ww * sizeof(T), wh, hipMemcpyHostToDevice, C->stream));
+    return MI355_OK;
+}
+
+template <typename T>
+int weights_write_host(size_t N, const T* uu, const T* vv, const T* weight, T cell, int G,
+        size_t* num_skipped, T* grid)
+{
+    TRY(check_weights_args(N, G));
+    if (!num_skipped || !grid) return fail(MI355_ERR_INVALID_ARG, "NULL output pointer");
+    if (N && (!uu || !vv || !weight)) return fail(MI355_ERR_INVALID_ARG, "NULL input array");
+    if (N == 0) { *num_skipped = 0; return ok(); }   // oskar_grid_weights.c:116
+    std::lock_guard<std::mutex> lock(g_default_lock);
+    mi355_ctx* C = nullptr;
+    TRY(get_default_ctx(&C));
+    HIP_TRY(hipSetDevice(C->device));
+    TRY(ensure(C->in_uu, N * sizeof(T)));
+    TRY(ensure(C->in_vv, N * sizeof(T)));
+    TRY(ensure(C->in_weight, N * sizeof(T)));
+    TRY(ensure(C->in_skipped, sizeof(unsigned long long)));
+    TRY(ensure(C->scalars, SC_COUNT * sizeof(unsigned long long)));
+    HIP_TRY(hipMemcpyAsync(C->in_uu.p, uu, N * sizeof(T), hipMemcpyHostToDevice, C->stream));
+    HIP_TRY(hipMemcpyAsync(C->in_vv.p, vv, N * sizeof(T), hipMemcpyHostToDevice, C->stream));
+    HIP_TRY(hipMemcpyAsync(C->in_weight.p, weight, N * sizeof(T), hipMemcpyHostToDevice, C->stream));
+    unsigned long long* sc = C->scalars.as<unsigned long long>();
+    int* box = reinterpret_cast<int*>(sc + SC_W_BOX);
+    unsigned long long* skipped = C->in_skipped.as<unsigned long long>();
+    hipLaunchKernelGGL(k_weight_init, dim3(1), dim3(64), 0, C->stream, sc + SC_W_COUNT,
+            (unsigned long long)N, skipped, box);
+    const T gs = (T)G * cell;
+    int cur = 0;
+    TRY(weights_sorted_cells<T>(C, N, C->in_uu.as<T>(), C->in_vv.as<T>(), gs, G, skipped, box,
+            &cur));
+    int hbox[4];
+    unsigned long long hskip = 0;
+    HIP_TRY(hipMemcpyAsync(hbox, box, sizeof(hbox), hipMemcpyDeviceToHost, C->stream));
+    HIP_TRY(hipMemcpyAsync(&hskip, skipped, sizeof(hskip), hipMemcpyDeviceToHost, C->stream));
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    if (hbox[0] <= hbox[2])
+    {
+        TRY(upload_weight_window<T>(C, grid, G, hbox));
+        const size_t ww = (size_t)(hbox[2] - hbox[0] + 1), wh = (size_t)(hbox[3] - hbox[1] + 1);
+        hipLaunchKernelGGL(k_weight_sum<T>, dim3(grid_blocks(N, 256, 65535)), dim3(256), 0,
+                C->stream, C->keys[cur].as<unsigned>(), C->vals[cur].as<unsigned>(),
+                C->in_weight.as<T>(), (unsigned long long)N,
+                (unsigned)((unsigned long long)G * G), G, C->in_grid.as<T>(), hbox[0], hbox[1],
+                (unsigned long long)ww);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpy2DAsync(grid + (size_t)hbox[1] * G + hbox[0], (size_t)G * sizeof(T),
+                C->in_grid.p, ww * sizeof(T), ww * sizeof(T), wh, hipMemcpyDeviceToHost,
+                C->stream));
+        HIP_TRY(hipStreamSynchronize(C->stream));
+    }
+    *num_skipped = (size_t)hskip;
+    return ok();
+}
+
+template <typename T>
+int weights_read_host(size_t N, const T* uu, const T* vv, const T* weight_in, T* weight_out,
+        T cell, int G, size_t* num_skipped, const T* grid)
+{
+    TRY(check_weights_args(N, G));
+    if (!num_skipped || !grid) return fail(MI355_ERR_INVALID_ARG, "NULL pointer argument");
+    if (N && (!uu || !vv || !weight_in || !weight_out))
+        return fail(MI355_ERR_INVALID_ARG, "NULL point array");
+    if (N == 0) { *num_skipped = 0; return ok(); }   // oskar_grid_weights.c:149
+    std::lock_guard<std::mutex> lock(g_default_lock);
+    mi355_ctx* C = nullptr;
+    TRY(get_default_ctx(&C));
+    HIP_TRY(hipSetDevice(C->device));
+    TRY(ensure(C->in_uu, N * sizeof(T)));
+    TRY(ensure(C->in_vv, N * sizeof(T)));
+    TRY(ensure(C->in_weight, N * sizeof(T)));
+    TRY(ensure(C->in_weight_out, N * sizeof(T)));
+    TRY(ensure(C->in_skipped, sizeof(unsigned long long)));
+    TRY(ensure(C->scalars, SC_COUNT * sizeof(unsigned long long)));
+    HIP_TRY(hipMemcpyAsync(C->in_uu.p, uu, N * sizeof(T), hipMemcpyHostToDevice, C->stream));
+    HIP_TRY(hipMemcpyAsync(C->in_vv.p, vv, N * sizeof(T), hipMemcpyHostToDevice, C->stream));
+    unsigned long long* sc = C->scalars.as<unsigned long long>();
+    int* box = reinterpret_cast<int*>(sc + SC_W_BOX);
+    unsigned long long* skipped = C->in_skipped.as<unsigned long long>();
+    hipLaunchKernelGGL(k_weight_init, dim3(1), dim3(64), 0, C->stream,
+            (unsigned long long*)nullptr, 0ull, skipped, box);
+    const T gs = (T)G * cell;
+    hipLaunchKernelGGL(k_weight_cells<T>, dim3(grid_blocks(N, 256, 16384)), dim3(256), 0,
+            C->stream, C->in_uu.as<T>(), C->in_vv.as<T>(), (unsigned long long)N, gs, G, 0u,
+            (unsigned*)nullptr, (unsigned*)nullptr, skipped, box);
+    HIP_TRY(hipGetLastError());
+    int hbox[4];
+    unsigned long long hskip = 0;
+    HIP_TRY(hipMemcpyAsync(hbox, box, sizeof(hbox), hipMemcpyDeviceToHost, C->stream));
+    HIP_TRY(hipMemcpyAsync(&hskip, skipped, sizeof(hskip), hipMemcpyDeviceToHost, C->stream));
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    if (hbox[0] <= hbox[2])
+    {
+        HIP_TRY(hipMemcpyAsync(C->in_weight.p, weight_in, N * sizeof(T), hipMemcpyHostToDevice,
+                C->stream));
+        // points off the grid leave their weight_out entries as they were
+        if (hskip)
+            HIP_TRY(hipMemcpyAsync(C->in_weight_out.p, weight_out, N * sizeof(T),
+                    hipMemcpyHostToDevice, C->stream));
+        TRY(upload_weight_window<T>(C, grid, G, hbox));
+        const size_t ww = (size_t)(hbox[2] - hbox[0] + 1);
+        hipLaunchKernelGGL(k_weight_read<T>, dim3(grid_blocks(N, 256, 16384)), dim3(256), 0,
+                C->stream, C->in_uu.as<T>(), C->in_vv.as<T>(), C->in_weight.as<T>(),
+                C->in_weight_out.as<T>(), (unsigned long long)N, gs, G, C->in_grid.as<T>(),
+                hbox[0], hbox[1], (unsigned long long)ww, (unsigned long long*)nullptr);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(weight_out, C->in_weight_out.p, N * sizeof(T),
+                hipMemcpyDeviceToHost, C->stream));
+        HIP_TRY(hipStreamSynchronize(C->stream));
+    }
+    *num_skipped = (size_t)hskip;
+    return ok();
+}
+
 } // namespace
 
 // ===========================================================================
@@ -1167,7 +1391,7 @@ MI355_EXPORT int mi355_ctx_destroy(mi355_ctx* C)
         &C->run_z, &C->run_tb, &C->scratch,
         &C->jobs_d, &C->job_chunk_off_d, &C->job_vis_off_d, &C->job_tile_base_d,
         &C->job_out_d, &C->job_bbox_d, &C->in_conv, &C->in_uu, &C->in_vv, &C->in_ww,
-        &C->in_vis, &C->in_weight, &C->in_grid, &C->in_norm, &C->in_skipped};
+        &C->in_vis, &C->in_weight, &C->in_grid, &C->in_norm, &C->in_skipped, &C->in_weight_out};
     for (DevBuf* b : all) release(*b);
     for (int i = 0; i < EV_COUNT; ++i) if (C->ev[i]) (void)hipEventDestroy(C->ev[i]);
     delete C;
@@ -1500,6 +1724,75 @@ MI355_EXPORT int mi355_autotune_f_batched_dev(mi355_ctx* C, size_t P, const int*
     if (st != MI355_OK) return st;
     if (best_ms) *best_ms = best;
     return ok();
+}
+
+// ---- uniform-weighting pre-pass ------------------------------------------
+MI355_EXPORT void mi355_grid_weights_write_d(const size_t num_points, const double* uu,
+        const double* vv, const double* weight, const double cell_size_rad,
+        const int grid_size, size_t* num_skipped, double* grid)
+{
+    (void)weights_write_host<double>(num_points, uu, vv, weight, cell_size_rad, grid_size,
+            num_skipped, grid);
+}
+
+MI355_EXPORT void mi355_grid_weights_write_f(const size_t num_points, const float* uu,
+        const float* vv, const float* weight, const float cell_size_rad,
+        const int grid_size, size_t* num_skipped, float* grid)
+{
+    (void)weights_write_host<float>(num_points, uu, vv, weight, cell_size_rad, grid_size,
+            num_skipped, grid);
+}
+
+MI355_EXPORT void mi355_grid_weights_read_d(const size_t num_points, const double* uu,
+        const double* vv, const double* weight_in, double* weight_out,
+        const double cell_size_rad, const int grid_size, size_t* num_skipped,
+        const double* grid)
+{
+    (void)weights_read_host<double>(num_points, uu, vv, weight_in, weight_out, cell_size_rad,
+            grid_size, num_skipped, grid);
+}
+
+MI355_EXPORT void mi355_grid_weights_read_f(const size_t num_points, const float* uu,
+        const float* vv, const float* weight_in, float* weight_out,
+        const float cell_size_rad, const int grid_size, size_t* num_skipped,
+        const float* grid)
+{
+    (void)weights_read_host<float>(num_points, uu, vv, weight_in, weight_out, cell_size_rad,
+            grid_size, num_skipped, grid);
+}
+
+MI355_EXPORT int mi355_grid_weights_write_f_dev(mi355_ctx* C, size_t N, const float* uu,
+        const float* vv, const float* weight, float cell, int G,
+        unsigned long long* num_skipped, float* grid)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    return weights_write_dev<float>(C, N, uu, vv, weight, cell, G, num_skipped, grid);
+}
+
+MI355_EXPORT int mi355_grid_weights_write_d_dev(mi355_ctx* C, size_t N, const double* uu,
+        const double* vv, const double* weight, double cell, int G,
+        unsigned long long* num_skipped, double* grid)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    return weights_write_dev<double>(C, N, uu, vv, weight, cell, G, num_skipped, grid);
+}
+
+MI355_EXPORT int mi355_grid_weights_read_f_dev(mi355_ctx* C, size_t N, const float* uu,
+        const float* vv, const float* weight_in, float* weight_out, float cell, int G,
+        unsigned long long* num_skipped, const float* grid)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    return weights_read_dev<float>(C, N, uu, vv, weight_in, weight_out, cell, G, num_skipped,
+            grid);
+}
+
+MI355_EXPORT int mi355_grid_weights_read_d_dev(mi355_ctx* C, size_t N, const double* uu,
+        const double* vv, const double* weight_in, double* weight_out, double cell, int G,
+        unsigned long long* num_skipped, const double* grid)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    return weights_read_dev<double>(C, N, uu, vv, weight_in, weight_out, cell, G, num_skipped,
+            grid);
 }
 
 } // extern "C"

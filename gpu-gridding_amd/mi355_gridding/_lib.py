@@ -102,6 +102,14 @@ SIGNATURES = {
                                                  P, P, P, P, P]),
     "mi355_autotune_f_batched_dev": (I, [P, SZ, P, I, I, P, I, ctypes.POINTER(JobF), P, P, P, P, P,
                                          ctypes.POINTER(D)]),
+    "mi355_grid_weights_write_d": (None, [SZ, P, P, P, D, I, P, P]),
+    "mi355_grid_weights_write_f": (None, [SZ, P, P, P, F, I, P, P]),
+    "mi355_grid_weights_read_d": (None, [SZ, P, P, P, P, D, I, P, P]),
+    "mi355_grid_weights_read_f": (None, [SZ, P, P, P, P, F, I, P, P]),
+    "mi355_grid_weights_write_f_dev": (I, [P, SZ, P, P, P, F, I, P, P]),
+    "mi355_grid_weights_write_d_dev": (I, [P, SZ, P, P, P, D, I, P, P]),
+    "mi355_grid_weights_read_f_dev": (I, [P, SZ, P, P, P, P, F, I, P, P]),
+    "mi355_grid_weights_read_d_dev": (I, [P, SZ, P, P, P, P, D, I, P, P]),
 }
 
 _lib = None

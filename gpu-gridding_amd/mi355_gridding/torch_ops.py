@@ -278,3 +278,52 @@ def autotune_batched(ks, jobs, uu, vv, ww, vis, weight, ctx=None):
     _batched_call(_lib.load().mi355_autotune_f_batched_dev, ks, jobs, uu, vv, ww, vis, weight,
                   ctx, ctypes.byref(best))
     return best.value
+
+
+def _weights_fn(real, write):
+    lib = _lib.load()
+    if real == torch.float32:
+        return (lib.mi355_grid_weights_write_f_dev if write else
+                lib.mi355_grid_weights_read_f_dev), ctypes.c_float
+    if real == torch.float64:
+        return (lib.mi355_grid_weights_write_d_dev if write else
+                lib.mi355_grid_weights_read_d_dev), ctypes.c_double
+    raise ValueError("weights: float32 or float64 tensors")
+
+
+def grid_weights_write(uu, vv, weight, cell_size_rad, grid_size, grid, num_skipped, ctx=None):
+    """Uniform-weighting density: grid (real [G, G], precision of uu) += the
+    weights per cell, in point order (mi355_grid_weights_write_*_dev);
+    num_skipped (int64 tensor [1]) is set."""
+    ctx = ctx or context()
+    n = uu.numel()
+    real = uu.dtype
+    for t, name in ((uu, "uu"), (vv, "vv"), (weight, "weight")):
+        _check_dev(t, real, n, name)
+    G = int(grid_size)
+    _check_dev(grid, real, G * G, "grid")
+    _check_dev(num_skipped, torch.int64, 1, "num_skipped")
+    fn, scal = _weights_fn(real, True)
+    _lib.check(fn(ctx.handle, n, _dp(uu), _dp(vv), _dp(weight), scal(float(cell_size_rad)), G,
+                  _dp(num_skipped), _dp(grid)))
+    return grid
+
+
+def grid_weights_read(uu, vv, weight_in, weight_out, cell_size_rad, grid_size, grid,
+                      num_skipped, ctx=None):
+    """weight_out = weight_in / (cell total) for points on the grid (0 where the
+    total is 0; entries of points off the grid are left as they are)
+    (mi355_grid_weights_read_*_dev); num_skipped (int64 tensor [1]) is set."""
+    ctx = ctx or context()
+    n = uu.numel()
+    real = uu.dtype
+    for t, name in ((uu, "uu"), (vv, "vv"), (weight_in, "weight_in"),
+                    (weight_out, "weight_out")):
+        _check_dev(t, real, n, name)
+    G = int(grid_size)
+    _check_dev(grid, real, G * G, "grid")
+    _check_dev(num_skipped, torch.int64, 1, "num_skipped")
+    fn, scal = _weights_fn(real, False)
+    _lib.check(fn(ctx.handle, n, _dp(uu), _dp(vv), _dp(weight_in), _dp(weight_out),
+                  scal(float(cell_size_rad)), G, _dp(num_skipped), _dp(grid)))
+    return weight_out

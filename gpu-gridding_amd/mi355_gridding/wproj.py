@@ -8,6 +8,11 @@ reference, ``grid`` is accumulated in place and ``norm`` is accumulated; the
 C functions return num_skipped and the updated norm through pointers, which
 these wrappers return as ``(num_skipped, norm)``.
 
+``grid_weights_write_f/_d`` and ``grid_weights_read_f/_d`` do the same for
+the uniform-weighting pre-pass (oskar_grid_weights.h:60-135): the weights
+grid (float [G, G]) is accumulated in place, weight_out is written in place,
+and num_skipped is returned.
+
 Errors the reference would abort on (oskar_grid_wproj_gpu.cu:1833-1843,
 gpu_support.hpp:143-151) raise :class:`GridderError` here.
 """
@@ -96,3 +101,71 @@ def set_default_option(key, value):
     h = ctypes.c_void_p()
     _lib.check(lib.mi355_default_ctx(ctypes.byref(h)))
     _lib.check(lib.mi355_ctx_set_option(h, int(key), int(value)))
+
+
+def _weights_arrays(real, num_points, uu, vv, grid_size, grid):
+    N = int(num_points)
+    G = int(grid_size)
+    uu = _arr(uu, real, N, "uu")
+    vv = _arr(vv, real, N, "vv")
+    if not isinstance(grid, np.ndarray) or not grid.flags.c_contiguous or grid.dtype != real:
+        raise ValueError("grid must be a C-contiguous %s numpy array" % np.dtype(real).name)
+    if grid.size != G * G:
+        raise ValueError("grid: expected grid_size^2 values")
+    return N, G, uu, vv
+
+
+def _weights_write(fn, real, num_points, uu, vv, weight, cell_size_rad, grid_size, grid):
+    N, G, uu, vv = _weights_arrays(real, num_points, uu, vv, grid_size, grid)
+    weight = _arr(weight, real, N, "weight")
+    skipped = ctypes.c_size_t(0)
+    scal = ctypes.c_float if real == np.float32 else ctypes.c_double
+    fn(N, _ptr(uu), _ptr(vv), _ptr(weight), scal(cell_size_rad), G, ctypes.byref(skipped),
+       _ptr(grid))
+    _lib.check_last()
+    return skipped.value
+
+
+def _weights_read(fn, real, num_points, uu, vv, weight_in, weight_out, cell_size_rad,
+                  grid_size, grid):
+    N, G, uu, vv = _weights_arrays(real, num_points, uu, vv, grid_size, grid)
+    weight_in = _arr(weight_in, real, N, "weight_in")
+    if not isinstance(weight_out, np.ndarray) or not weight_out.flags.c_contiguous or \
+            weight_out.dtype != real or weight_out.size != N:
+        raise ValueError("weight_out must be a C-contiguous %s array of num_points values"
+                         % np.dtype(real).name)
+    skipped = ctypes.c_size_t(0)
+    scal = ctypes.c_float if real == np.float32 else ctypes.c_double
+    fn(N, _ptr(uu), _ptr(vv), _ptr(weight_in), _ptr(weight_out), scal(cell_size_rad), G,
+       ctypes.byref(skipped), _ptr(grid))
+    _lib.check_last()
+    return skipped.value
+
+
+def grid_weights_write_f(num_points, uu, vv, weight, cell_size_rad, grid_size, grid):
+    """Mirrors oskar_grid_weights_write_f (oskar_grid_weights.h:107-111):
+    grid (float32 [G, G]) += weights per cell; returns num_skipped."""
+    return _weights_write(_lib.load().mi355_grid_weights_write_f, np.float32, num_points, uu,
+                          vv, weight, cell_size_rad, grid_size, grid)
+
+
+def grid_weights_write_d(num_points, uu, vv, weight, cell_size_rad, grid_size, grid):
+    """Mirrors oskar_grid_weights_write_d (oskar_grid_weights.h:60-64)."""
+    return _weights_write(_lib.load().mi355_grid_weights_write_d, np.float64, num_points, uu,
+                          vv, weight, cell_size_rad, grid_size, grid)
+
+
+def grid_weights_read_f(num_points, uu, vv, weight_in, weight_out, cell_size_rad, grid_size,
+                        grid):
+    """Mirrors oskar_grid_weights_read_f (oskar_grid_weights.h:131-135):
+    weight_out[i] = weight_in[i] / grid[cell] (0 where that is 0) for points
+    on the grid; returns num_skipped."""
+    return _weights_read(_lib.load().mi355_grid_weights_read_f, np.float32, num_points, uu, vv,
+                         weight_in, weight_out, cell_size_rad, grid_size, grid)
+
+
+def grid_weights_read_d(num_points, uu, vv, weight_in, weight_out, cell_size_rad, grid_size,
+                        grid):
+    """Mirrors oskar_grid_weights_read_d (oskar_grid_weights.h:84-88)."""
+    return _weights_read(_lib.load().mi355_grid_weights_read_d, np.float64, num_points, uu, vv,
+                         weight_in, weight_out, cell_size_rad, grid_size, grid)

@@ -248,6 +248,62 @@ MI355_EXPORT int mi355_autotune_f_batched_dev(mi355_ctx* ctx,
         const mi355_job_f* jobs, const float* uu, const float* vv,
         const float* ww, const float* vis, const float* weight, double* best_ms);
 
+/* ------------------------------------------------------------------------
+ * Uniform-weighting pre-pass.
+ *
+ * Write adds each point's weight to the cell of the weights grid (a G x G
+ * real array, row v, column u) that the point falls in; read divides each
+ * point's weight by its cell's total (0 where the total is 0).  Points whose
+ * cell lies off the grid are counted in num_skipped: write ignores them and
+ * read leaves their weight_out entries untouched.  The grid written is
+ * bit-identical to the serial one (per cell, the same additions in point
+ * order).  grid_size is limited to 65535 and num_points to 2^32-1 per call
+ * (MI355_ERR_TOO_LARGE).
+ * ---------------------------------------------------------------------- */
+
+/* Replace oskar_grid_weights_write_d / _f (oskar_grid_weights.h:60-64 /
+ * :107-111; called at main.cpp:229-237); same arguments, same meaning.  Only
+ * the window of the grid spanned by the points' cells is copied to the
+ * device and back.  Errors: mi355_grid_last_error. */
+MI355_EXPORT void mi355_grid_weights_write_d(const size_t num_points,
+        const double* uu, const double* vv, const double* weight,
+        const double cell_size_rad, const int grid_size,
+        size_t* num_skipped, double* grid);
+MI355_EXPORT void mi355_grid_weights_write_f(const size_t num_points,
+        const float* uu, const float* vv, const float* weight,
+        const float cell_size_rad, const int grid_size,
+        size_t* num_skipped, float* grid);
+
+/* Replace oskar_grid_weights_read_d / _f (oskar_grid_weights.h:84-88 /
+ * :131-135; called at main.cpp:273-285); same arguments, same meaning. */
+MI355_EXPORT void mi355_grid_weights_read_d(const size_t num_points,
+        const double* uu, const double* vv, const double* weight_in,
+        double* weight_out, const double cell_size_rad, const int grid_size,
+        size_t* num_skipped, const double* grid);
+MI355_EXPORT void mi355_grid_weights_read_f(const size_t num_points,
+        const float* uu, const float* vv, const float* weight_in,
+        float* weight_out, const float cell_size_rad, const int grid_size,
+        size_t* num_skipped, const float* grid);
+
+/* Device-pointer forms of the four calls above, asynchronous on the
+ * context's stream and free of host synchronisation.  *num_skipped (a
+ * device u64) is set, not accumulated, like the reference's; the grid
+ * accumulates across write calls. */
+MI355_EXPORT int mi355_grid_weights_write_f_dev(mi355_ctx* ctx, size_t num_points,
+        const float* uu, const float* vv, const float* weight, float cell_size_rad,
+        int grid_size, unsigned long long* num_skipped, float* grid);
+MI355_EXPORT int mi355_grid_weights_write_d_dev(mi355_ctx* ctx, size_t num_points,
+        const double* uu, const double* vv, const double* weight, double cell_size_rad,
+        int grid_size, unsigned long long* num_skipped, double* grid);
+MI355_EXPORT int mi355_grid_weights_read_f_dev(mi355_ctx* ctx, size_t num_points,
+        const float* uu, const float* vv, const float* weight_in, float* weight_out,
+        float cell_size_rad, int grid_size, unsigned long long* num_skipped,
+        const float* grid);
+MI355_EXPORT int mi355_grid_weights_read_d_dev(mi355_ctx* ctx, size_t num_points,
+        const double* uu, const double* vv, const double* weight_in, double* weight_out,
+        double cell_size_rad, int grid_size, unsigned long long* num_skipped,
+        const double* grid);
+
 #ifdef __cplusplus
 }
 #endif

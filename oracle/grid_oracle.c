@@ -29,6 +29,14 @@
  *   oracle_grid_wproj_fsel_d_tiles  the same FP64 truth restricted to a list
  *                              of 64x64 grid tiles, multi-threaded, for
  *                              spot checks of full-size grids.
+ *
+ * Uniform weighting (the pre-pass upstream of the gridder), restated from
+ * the reference's serial functions, same precision and order:
+ *   oracle_grid_weights_write_f / _d   oskar_grid_weights.c:106-134 / :37-65
+ *   oracle_grid_weights_read_f / _d    oskar_grid_weights.c:136-165 / :67-96
+ * pinned against the same functions compiled from the reference (oracle/_ref)
+ * and by closed forms (unit weights give per-cell counts; read after write
+ * gives weight / cell total).
  */
 #include <math.h>
 #include <stdint.h>
@@ -440,4 +448,84 @@ ORACLE_API void oracle_grid_wproj_fsel_d_tiles(size_t num_w_planes,
     for (t = 0; t < num_threads; ++t)
         pthread_join(th[t], NULL);
     pthread_mutex_destroy(&J.lock);
+}
+
+
+/* ------------------------------------------------------------------------
+ * Uniform weighting: cell selection of oskar_grid_weights.c (:117-128 for
+ * float, :48-59 for double), then the per-cell sum in visibility order
+ * (write) or the division by the cell total (read).
+ * ---------------------------------------------------------------------- */
+static int weights_cell_f(float u, float v, float grid_scale, int grid_size, size_t* t)
+{
+    const int centre = grid_size / 2;
+    const int grid_u = (int)roundf(-u * grid_scale) + centre;
+    const int grid_v = (int)roundf(v * grid_scale) + centre;
+    *t = (size_t)grid_v * (size_t)grid_size + (size_t)grid_u;
+    return !(grid_u >= grid_size || grid_u < 0 || grid_v >= grid_size || grid_v < 0);
+}
+
+static int weights_cell_d(double u, double v, double grid_scale, int grid_size, size_t* t)
+{
+    const int centre = grid_size / 2;
+    const int grid_u = (int)round(-u * grid_scale) + centre;
+    const int grid_v = (int)round(v * grid_scale) + centre;
+    *t = (size_t)grid_v * (size_t)grid_size + (size_t)grid_u;
+    return !(grid_u >= grid_size || grid_u < 0 || grid_v >= grid_size || grid_v < 0);
+}
+
+ORACLE_API void oracle_grid_weights_write_f(size_t n, const float* uu, const float* vv,
+        const float* weight, float cell_size_rad, int grid_size, size_t* num_skipped,
+        float* grid)
+{
+    const float grid_scale = grid_size * cell_size_rad;
+    size_t t;
+    *num_skipped = 0;
+    for (size_t i = 0; i < n; ++i)
+    {
+        if (!weights_cell_f(uu[i], vv[i], grid_scale, grid_size, &t)) { ++*num_skipped; continue; }
+        grid[t] += weight[i];
+    }
+}
+
+ORACLE_API void oracle_grid_weights_write_d(size_t n, const double* uu, const double* vv,
+        const double* weight, double cell_size_rad, int grid_size, size_t* num_skipped,
+        double* grid)
+{
+    const double grid_scale = grid_size * cell_size_rad;
+    size_t t;
+    *num_skipped = 0;
+    for (size_t i = 0; i < n; ++i)
+    {
+        if (!weights_cell_d(uu[i], vv[i], grid_scale, grid_size, &t)) { ++*num_skipped; continue; }
+        grid[t] += weight[i];
+    }
+}
+
+ORACLE_API void oracle_grid_weights_read_f(size_t n, const float* uu, const float* vv,
+        const float* weight_in, float* weight_out, float cell_size_rad, int grid_size,
+        size_t* num_skipped, const float* grid)
+{
+    const float grid_scale = grid_size * cell_size_rad;
+    size_t t;
+    *num_skipped = 0;
+    for (size_t i = 0; i < n; ++i)
+    {
+        if (!weights_cell_f(uu[i], vv[i], grid_scale, grid_size, &t)) { ++*num_skipped; continue; }
+        weight_out[i] = (grid[t] != 0.0) ? weight_in[i] / grid[t] : 0.0f;
+    }
+}
+
+ORACLE_API void oracle_grid_weights_read_d(size_t n, const double* uu, const double* vv,
+        const double* weight_in, double* weight_out, double cell_size_rad, int grid_size,
+        size_t* num_skipped, const double* grid)
+{
+    const double grid_scale = grid_size * cell_size_rad;
+    size_t t;
+    *num_skipped = 0;
+    for (size_t i = 0; i < n; ++i)
+    {
+        if (!weights_cell_d(uu[i], vv[i], grid_scale, grid_size, &t)) { ++*num_skipped; continue; }
+        weight_out[i] = (grid[t] != 0.0) ? weight_in[i] / grid[t] : 0.0;
+    }
 }

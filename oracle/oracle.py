@@ -25,8 +25,21 @@ D = ctypes.c_double
 _GRID_F = [SZ, P, I, I, P, SZ, P, P, P, P, P, F, F, I, P, P, P]
 _GRID_D = [SZ, P, I, I, P, SZ, P, P, P, P, P, D, D, I, P, P, P]
 
+_W_WRITE_F = [SZ, P, P, P, F, I, P, P]
+_W_WRITE_D = [SZ, P, P, P, D, I, P, P]
+_W_READ_F = [SZ, P, P, P, P, F, I, P, P]
+_W_READ_D = [SZ, P, P, P, P, D, I, P, P]
+
 _oracle = None
 _ref = None
+
+
+def _weights_sigs(lib, prefix):
+    for suffix, sig in (("write_f", _W_WRITE_F), ("write_d", _W_WRITE_D),
+                        ("read_f", _W_READ_F), ("read_d", _W_READ_D)):
+        fn = getattr(lib, prefix + suffix)
+        fn.argtypes = sig
+        fn.restype = None
 
 
 def oracle():
@@ -48,6 +61,7 @@ def oracle():
                    lib.oracle_select_f, lib.oracle_select_d, lib.oracle_degrid_wproj_fsel_d,
                    lib.oracle_grid_wproj_fsel_d_tiles):
             fn.restype = None
+        _weights_sigs(lib, "oracle_grid_weights_")
         _oracle = lib
     return _oracle
 
@@ -63,6 +77,7 @@ def ref():
         lib.oskar_grid_wproj_d.restype = None
         lib.check_value_float.argtypes = [F, F]
         lib.check_value_float.restype = I
+        _weights_sigs(lib, "oskar_grid_weights_")
         _ref = lib
     return _ref
 
@@ -191,3 +206,44 @@ def tiles_fsel_d(case, tiles_xy, tile=64, threads=8):
         ctypes.c_float(float(case["cell_size_rad"])), ctypes.c_float(float(case["w_scale"])),
         int(case["grid_size"]), int(tile), _p(t), t.shape[0], int(threads), _p(out))
     return out
+
+
+def weights_write(uu, vv, weight, cell_size_rad, grid_size, grid, use_ref=False):
+    """oskar_grid_weights_write_f / _d semantics (precision from uu's dtype):
+    grid ([G, G] real) += weights per cell in visibility order; returns
+    num_skipped."""
+    real = np.float32 if np.asarray(uu).dtype == np.float32 else np.float64
+    lib = ref() if use_ref else oracle()
+    if lib is None:
+        raise RuntimeError("compiled reference not built (make -C oracle ref)")
+    fn = (lib.oskar_grid_weights_write_f if use_ref else lib.oracle_grid_weights_write_f) \
+        if real == np.float32 else \
+        (lib.oskar_grid_weights_write_d if use_ref else lib.oracle_grid_weights_write_d)
+    uu, vv, weight = (np.ascontiguousarray(x, real) for x in (uu, vv, weight))
+    assert grid.dtype == real and grid.flags.c_contiguous
+    sk = ctypes.c_size_t(0)
+    scal = ctypes.c_float if real == np.float32 else ctypes.c_double
+    fn(uu.size, _p(uu), _p(vv), _p(weight), scal(float(cell_size_rad)), int(grid_size),
+       ctypes.byref(sk), _p(grid))
+    return sk.value
+
+
+def weights_read(uu, vv, weight_in, weight_out, cell_size_rad, grid_size, grid, use_ref=False):
+    """oskar_grid_weights_read_f / _d semantics: weight_out[i] = weight_in[i]
+    / grid[cell] (0 where the cell total is 0) for points inside the grid;
+    returns num_skipped."""
+    real = np.float32 if np.asarray(uu).dtype == np.float32 else np.float64
+    lib = ref() if use_ref else oracle()
+    if lib is None:
+        raise RuntimeError("compiled reference not built (make -C oracle ref)")
+    fn = (lib.oskar_grid_weights_read_f if use_ref else lib.oracle_grid_weights_read_f) \
+        if real == np.float32 else \
+        (lib.oskar_grid_weights_read_d if use_ref else lib.oracle_grid_weights_read_d)
+    uu, vv, weight_in = (np.ascontiguousarray(x, real) for x in (uu, vv, weight_in))
+    grid = np.ascontiguousarray(grid, real)
+    assert weight_out.dtype == real and weight_out.flags.c_contiguous
+    sk = ctypes.c_size_t(0)
+    scal = ctypes.c_float if real == np.float32 else ctypes.c_double
+    fn(uu.size, _p(uu), _p(vv), _p(weight_in), _p(weight_out), scal(float(cell_size_rad)),
+       int(grid_size), ctypes.byref(sk), _p(grid))
+    return sk.value

@@ -25,8 +25,13 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: full-size workload")
 
 
-def golden_names():
-    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+def golden_names(prefix=""):
+    """Fixture names: the gridding fixtures by default, the weighting ones
+    (weights_*.npz) with prefix="weights_"."""
+    names = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+    if prefix:
+        return [n for n in names if n.startswith(prefix)]
+    return [n for n in names if not n.startswith("weights_")]
 
 
 def load_golden(name):
@@ -57,6 +62,18 @@ def same_bits(a, b):
     a = np.ascontiguousarray(a)
     b = np.ascontiguousarray(b)
     return a.shape == b.shape and a.dtype == b.dtype and a.tobytes() == b.tobytes()
+
+
+def same_bits_or_nan(a, b):
+    """Bitwise equality of the non-NaN values and NaN in the same places (the
+    sign and payload of a NaN that an invalid operation creates differ
+    between x86 and the GPU; IEEE leaves them open)."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    if a.shape != b.shape or a.dtype != b.dtype:
+        return False
+    na, nb = np.isnan(a), np.isnan(b)
+    return np.array_equal(na, nb) and a[~na].tobytes() == b[~nb].tobytes()
 
 
 def same_scalar(a, b):

@@ -7,7 +7,9 @@ the selection rules) are run through oskar_grid_wproj_f and
 oskar_grid_wproj_d compiled unmodified from the reference sources
 (oracle/_ref/libref.so, built by `make -C oracle ref` from
 /root/reference/oskar_grid_wproj.c), and inputs plus outputs are saved as
-.npz.  Run from the repository root:
+.npz.  The weights_*.npz fixtures do the same for the uniform-weighting
+pre-pass (oskar_grid_weights_write/read_f/_d, from
+/root/reference/utils/oskar_grid_weights.c).  Run from the repository root:
 
     make -C oracle && python tests/golden/make_golden.py
 """
@@ -134,14 +136,89 @@ def save(name, d):
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
-def main():
-    save("random_g64", with_outputs(random_case(1, 400, 8, 6, 4, 64)))
-    save("random_g128_ov3", with_outputs(random_case(2, 600, 12, 5, 3, 128)))
-    save("special_g64", with_outputs(special_case()))
-    save("synth_g256", with_outputs(synth_case()))
-    save("accumulate_g64", with_outputs(random_case(3, 300, 8, 6, 4, 64), accumulate=True))
-    save("nonfinite_g64", with_outputs(nonfinite_case()))
+def weights_random_case(seed, N, G):
+    """Two blocks of points, a sixth of them off the grid, with cells whose
+    weights cancel to exactly zero and a few non-finite weights."""
+    rng = np.random.default_rng(seed)
+    cell = 1.0 / (G * 4.0)                 # grid_scale = 1/4 exactly
+    reach = (G / 2) * 4.0 * 1.2
+    c = dict(grid_size=np.int64(G), cell_size_rad=np.float64(cell))
+    for b in ("a", "b"):
+        uu = rng.uniform(-reach, reach, N)
+        vv = rng.uniform(-reach, reach, N)
+        w = rng.uniform(0.1, 3.0, N)
+        # exact cancellation: pairs of points in one cell with weights +x, -x
+        uu[:8] = uu[8:16]
+        vv[:8] = vv[8:16]
+        w[:8] = -w[8:16]
+        c["uu_" + b], c["vv_" + b], c["w_" + b] = uu, vv, w
+    # non-finite weights on the grid, in cells that block a also uses
+    for i, j, u, v, x in ((100, 300, 21.0, -9.0, np.inf), (200, 400, -33.0, 41.0, np.nan)):
+        c["uu_b"][i], c["vv_b"][i], c["w_b"][i] = u, v, x
+        c["uu_a"][j], c["vv_a"][j] = u + 0.5, v + 0.5
+    return c
+
+
+def weights_special_case(G):
+    """Cells on the rounding and grid edges (grid_scale = 1: position = -uu,
+    vv), non-finite and out-of-int coordinates, and signed zeros."""
+    h = G // 2
+    pts = []
+    for n in (0, 1, 2, -1, -2, h - 1, -h):
+        pts += [(-(n + 0.5), 0.0), (0.0, n + 0.5), (n - 0.5, -(n - 0.5))]
+    pts += [(-(G - 1 - h), 0.0), (-(G - h), 0.0), (h, 0.0), (h + 1, 0.0),
+            (0.0, G - 1 - h), (0.0, G - h), (0.0, -h), (0.0, -h - 1),
+            (-(G - 1 - h) - 0.49, 0.2), (-(G - 1 - h) - 0.5, 0.2), (h + 0.5, 0.1), (h + 0.49, 0.1)]
+    pts += [(np.nan, 0.0), (0.0, np.nan), (np.inf, 0.0), (0.0, -np.inf), (1e10, 0.0),
+            (-3e9, 0.0), (0.0, 2.2e9), (-0.0, -0.0), (0.0, -0.0), (-0.4, 0.4)]
+    a = np.array(pts, np.float64)
+    rng = np.random.default_rng(G)
+    w = rng.uniform(0.5, 2.0, a.shape[0])
+    return dict(grid_size=np.int64(G), cell_size_rad=np.float64(1.0 / G),
+                uu_a=a[:, 0], vv_a=a[:, 1], w_a=w,
+                uu_b=a[::-1, 0].copy(), vv_b=a[::-1, 1].copy(), w_b=w[::-1] * 0.5)
+
+
+def weights_with_outputs(c):
+    """write(a), write(b) into one zeroed grid, then read(a) into a weight_out
+    prefilled with -7 (entries of points off the grid stay -7)."""
+    if oracle.ref() is None:
+        raise SystemExit("oracle/_ref/libref.so not built (make -C oracle ref)")
+    G = int(c["grid_size"])
+    out = dict(c)
+    for tag, real in (("f", np.float32), ("d", np.float64)):
+        cell = real(c["cell_size_rad"])
+        grid = np.zeros((G, G), real)
+        sk_a = oracle.weights_write(c["uu_a"].astype(real), c["vv_a"].astype(real),
+                                    c["w_a"].astype(real), cell, G, grid, use_ref=True)
+        sk_b = oracle.weights_write(c["uu_b"].astype(real), c["vv_b"].astype(real),
+                                    c["w_b"].astype(real), cell, G, grid, use_ref=True)
+        wo = np.full(c["uu_a"].size, -7.0, real)
+        sk_r = oracle.weights_read(c["uu_a"].astype(real), c["vv_a"].astype(real),
+                                   c["w_a"].astype(real), wo, cell, G, grid, use_ref=True)
+        out.update({"grid_" + tag: grid, "weight_out_" + tag: wo,
+                    "skipped_" + tag: np.array([sk_a, sk_b, sk_r], np.int64)})
+    return out
+
+
+FIXTURES = {
+    "random_g64": lambda: with_outputs(random_case(1, 400, 8, 6, 4, 64)),
+    "random_g128_ov3": lambda: with_outputs(random_case(2, 600, 12, 5, 3, 128)),
+    "special_g64": lambda: with_outputs(special_case()),
+    "synth_g256": lambda: with_outputs(synth_case()),
+    "accumulate_g64": lambda: with_outputs(random_case(3, 300, 8, 6, 4, 64), accumulate=True),
+    "nonfinite_g64": lambda: with_outputs(nonfinite_case()),
+    "weights_random_g96": lambda: weights_with_outputs(weights_random_case(21, 3000, 96)),
+    "weights_special_g64": lambda: weights_with_outputs(weights_special_case(64)),
+    "weights_special_g63": lambda: weights_with_outputs(weights_special_case(63)),
+}
+
+
+def main(names):
+    """Writes the named fixtures (all of them when none are named)."""
+    for name in names or FIXTURES:
+        save(name, FIXTURES[name]())
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -143,3 +143,54 @@ def test_synth_tracks_are_time_major():
     r = np.hypot(v["uu"], v["vv"]).reshape(7, 30)
     assert np.allclose(r, r[0], rtol=1e-5)          # each baseline keeps its radius
     assert not np.allclose(v["uu"].reshape(7, 30), v["uu"][:30])   # and rotates
+
+
+def test_weights_drop_in_signatures_match_reference_header():
+    """Same parameter lists as oskar_grid_weights_write/read_f/_d
+    (oskar_grid_weights.h:60-64, :84-88, :107-111, :131-135)."""
+    text = open(HEADER).read()
+    for prec, real in (("f", "float"), ("d", "double")):
+        for op, want in (("write", ["size_t", real + "*", real + "*", real + "*", real, "int",
+                                    "size_t*", real + "*"]),
+                         ("read", ["size_t", real + "*", real + "*", real + "*", real + "*",
+                                   real, "int", "size_t*", real + "*"])):
+            m = re.search(r"void mi355_grid_weights_%s_%s\((.*?)\);" % (op, prec), text, re.S)
+            params = [p.strip() for p in m.group(1).replace("\n", " ").split(",")]
+            types = [re.sub(r"\s+", " ", p.rsplit(" ", 1)[0].replace("const ", "")).strip()
+                     for p in params]
+            assert types == want, (op, prec, types)
+
+
+def test_weights_invalid_sizes_rejected_before_device():
+    """grid_size outside [1, 65535] is refused before any array is read."""
+    lib = _lib.load()
+    u = np.zeros(3, np.float32)
+    g = np.zeros(1, np.float32)
+    p = ctypes.c_void_p(u.ctypes.data)
+    sk = ctypes.c_size_t(9)
+    for G, status in ((0, _lib.ERR_INVALID_ARG), (-5, _lib.ERR_INVALID_ARG),
+                      (65536, _lib.ERR_TOO_LARGE)):
+        lib.mi355_grid_weights_write_f(3, p, p, p, ctypes.c_float(0.01), G, ctypes.byref(sk),
+                                       ctypes.c_void_p(g.ctypes.data))
+        assert lib.mi355_grid_last_error() == status
+        lib.mi355_grid_weights_read_f(3, p, p, p, p, ctypes.c_float(0.01), G, ctypes.byref(sk),
+                                      ctypes.c_void_p(g.ctypes.data))
+        assert lib.mi355_grid_last_error() == status
+
+
+def test_weights_zero_points_and_nulls_without_device():
+    """oskar_grid_weights.c:116 / :149 zero num_skipped for an empty block."""
+    lib = _lib.load()
+    g = np.full((8, 8), 3.0, np.float32)
+    u = np.zeros(0, np.float32)
+    assert wproj.grid_weights_write_f(0, u, u, u, 0.01, 8, g) == 0
+    wo = np.zeros(0, np.float32)
+    assert wproj.grid_weights_read_f(0, u, u, u, wo, 0.01, 8, g) == 0
+    assert np.all(g == 3.0)
+    sk = ctypes.c_size_t(5)
+    lib.mi355_grid_weights_write_d(4, None, None, None, ctypes.c_double(0.1), 8,
+                                   ctypes.byref(sk), None)
+    assert lib.mi355_grid_last_error() == _lib.ERR_INVALID_ARG
+    lib.mi355_grid_weights_read_f(4, None, None, None, None, ctypes.c_float(0.1), 8,
+                                  ctypes.byref(sk), ctypes.c_void_p(g.ctypes.data))
+    assert lib.mi355_grid_last_error() == _lib.ERR_INVALID_ARG
