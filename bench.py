@@ -261,6 +261,17 @@ def main():
         ctx.set_option(_lib.OPT_COMPENSATED, 1)
         gc = torch.zeros_like(grid)
         ops.grid(*step_args, gc, torch.zeros_like(norm), skipped, ctx=ctx)
+        # time the compensated form on the same inputs (3 passes)
+        torch.cuda.synchronize()
+        c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        c0.record(stream)
+        for _ in range(3):
+            ops.grid(*step_args, gc, torch.zeros_like(norm), skipped, ctx=ctx)
+        c1.record(stream)
+        torch.cuda.synchronize()
+        comp_ms = c0.elapsed_time(c1) / 3
+        gc.zero_()
+        ops.grid(*step_args, gc, torch.zeros_like(norm), skipped, ctx=ctx)
         ctx.set_option(_lib.OPT_COMPENSATED, 1 if a.compensated else 0)
         torch.cuda.synchronize()
         m64 = g64.abs().max().item()
@@ -271,6 +282,7 @@ def main():
             return dict(max_rel=dd.abs().max().item() / m64,
                         l2_rel=torch.linalg.norm(dd).item() / l64)
         accuracy = dict(plain_fp32=err(g32), compensated_fp32=err(gc),
+                        compensated_ms_per_step=round(comp_ms, 3),
                         norm_rel=abs(n32.item() - n64.item()) / abs(n64.item()),
                         truth="FP64 accumulation, FP32 selection (mi355_grid_wproj_fsel_d_dev)")
         del g64, gc, g32
