@@ -20,6 +20,10 @@ FETCH_SIZE / WRITE_SIZE counter summary in profiles/, when present.
 from its sources into oracle/_ref, else the oracle's restatement) on one
 core on a prefix of the same visibilities.  `dropin` (with --dropin) times
 the host-pointer drop-in mi355_grid_wproj_f, host-device copies included.
+`weights` times the uniform-weighting pre-pass on the same points (device
+write of the 16384^2 weights grid, then the read of every point's weight),
+beside the reference's serial oskar_grid_weights_write_f / _read_f over all
+points on one core.
 
 --copy-calibration MIB only runs plain device copies of MIB MiB (the
 reference kernel for scaling the HBM byte counters) and exits.
@@ -74,7 +78,7 @@ def cpu_baseline(case, seconds):
 
 
 def bench_batched(a):
-    """Config 3: 64 snapshot grids (4096^2) with log-uniform visibility
+    """The batched workload: 64 snapshot grids (4096^2) with log-uniform visibility
     counts, autotuned, gridded as one captured HIP graph per step; compared
     with the same pass launched eagerly and with 64 sequential single-grid
     calls."""
@@ -139,13 +143,14 @@ def bench_batched(a):
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(graph_ms, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic (seeded generator, config-3 shapes)",
+        "dtype": "f32", "data": "synthetic (seeded generator, batched-64x4096 shapes)",
         "config": {"workload": "batched-64x4096", "num_jobs": len(jobs), "num_points": N,
                    "grid_size": 4096, "num_w_planes": b["num_w_planes"],
                    "max_support": b["max_support"],
                    "autotuned": {"grid_kernel": ctx.get_option(_lib.OPT_GRID_KERNEL),
                                  "visits_per_item": ctx.get_option(_lib.OPT_VISITS_PER_ITEM),
-                                 "min_stage_run": ctx.get_option(_lib.OPT_MIN_STAGE_RUN)}},
+                                 "min_stage_run": ctx.get_option(_lib.OPT_MIN_STAGE_RUN),
+                                 "item_group": ctx.get_option(_lib.OPT_ITEM_GROUP)}},
         "batched": {"graph_ms": round(graph_ms, 3), "eager_batched_ms": round(eager_ms, 3),
                     "sequential_64_ms": round(seq_ms, 3),
                     "graph_vs_sequential": round(seq_ms / graph_ms, 3),
@@ -155,6 +160,65 @@ def bench_batched(a):
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_weights(case, d, ctx, no_cpu, reps=5):
+    """Uniform-weighting pre-pass on the device (write, then read) over all
+    points of the workload, and the reference's serial functions on the
+    host over the same points."""
+    import numpy as np
+    import torch
+    from mi355_gridding import torch_ops as ops
+    N, G = case["num_points"], case["grid_size"]
+    cell = float(case["cell_size_rad"])
+    wgrid = torch.zeros((G, G), dtype=torch.float32, device="cuda")
+    wout = torch.empty_like(d["weight"])
+    sk = torch.zeros(1, dtype=torch.int64, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ms_w = timed(lambda: ops.grid_weights_write(d["uu"], d["vv"], d["weight"], cell, G, wgrid,
+                                                sk, ctx=ctx))
+    wgrid.zero_()
+    ops.grid_weights_write(d["uu"], d["vv"], d["weight"], cell, G, wgrid, sk, ctx=ctx)
+    ms_r = timed(lambda: ops.grid_weights_read(d["uu"], d["vv"], d["weight"], wout, cell, G,
+                                               wgrid, sk, ctx=ctx))
+    out = dict(ms_write=round(ms_w, 4), ms_read=round(ms_r, 4),
+               value=round(N / ((ms_w + ms_r) * 1e-3) / 1e6, 1), unit="Mpoints/s",
+               num_skipped=int(sk.item()),
+               note="device pointers; write = cell keys + stable radix sort + ordered "
+                    "per-cell sums into a zeroed FP32 grid; read = one division per point")
+    del wgrid, wout
+    if not no_cpu:
+        from oracle import oracle
+        use_ref = oracle.ref() is not None
+        hgrid = np.zeros((G, G), np.float32)
+        hout = np.empty(N, np.float32)
+        t0 = time.perf_counter()
+        oracle.weights_write(case["uu"], case["vv"], case["weight"], np.float32(cell), G, hgrid,
+                             use_ref=use_ref)
+        t1 = time.perf_counter()
+        oracle.weights_read(case["uu"], case["vv"], case["weight"], hout, np.float32(cell), G,
+                            hgrid, use_ref=use_ref)
+        t2 = time.perf_counter()
+        out["cpu_baseline"] = dict(
+            ms_write=round((t1 - t0) * 1e3, 1), ms_read=round((t2 - t1) * 1e3, 1),
+            value=round(N / (t2 - t0) / 1e6, 2), unit="Mpoints/s", cores=1,
+            kind="reference" if use_ref else "port",
+            sample="all %d points, serial oskar_grid_weights_write_f + _read_f" % N)
+        del hgrid, hout
+    log("weights %s" % json.dumps(out))
+    return out
 
 
 def main():
@@ -172,8 +236,12 @@ def main():
                          "tiles with tables staged in LDS / read directly")
     ap.add_argument("--visits-per-item", type=int, default=2048)
     ap.add_argument("--min-stage-run", type=int, default=6)
+    ap.add_argument("--item-group", type=int, default=8,
+                    help="strip kernel: consecutive work items dealt to one XCD")
     ap.add_argument("--dropin", action="store_true",
                     help="also time the host-pointer drop-in (PCIe included)")
+    ap.add_argument("--no-weights", action="store_true",
+                    help="skip the uniform-weighting pre-pass leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run plain device copies of MIB MiB, then exit")
     a = ap.parse_args()
@@ -217,6 +285,7 @@ def main():
     ctx.set_option(_lib.OPT_GRID_KERNEL, {"strips": 0, "staged": 2, "direct": 1}[a.kernel])
     ctx.set_option(_lib.OPT_VISITS_PER_ITEM, a.visits_per_item)
     ctx.set_option(_lib.OPT_MIN_STAGE_RUN, a.min_stage_run)
+    ctx.set_option(_lib.OPT_ITEM_GROUP, a.item_group)
 
     def step():
         ops.grid(ks, d["uu"], d["vv"], d["ww"], d["vis"], d["weight"], case["cell_size_rad"],
@@ -307,6 +376,8 @@ def main():
                            "host<->device inside the call (best of 2)")
         log("drop-in %s" % json.dumps(dropin))
 
+    weights = None if a.no_weights else bench_weights(case, d, ctx, a.no_cpu_baseline)
+
     traffic = None
     kname = {"strips": "k_grid_strips", "staged": "k_grid_staged", "direct": "k_grid_tiles"}[a.kernel]
     if os.path.exists(TRAFFIC_FILE):
@@ -336,7 +407,8 @@ def main():
                    "num_w_planes": case["num_w_planes"], "max_support": case["max_support"],
                    "oversample": case["oversample"],
                    "accumulation": "compensated" if a.compensated else "plain",
-                   "tile_kernel": a.kernel, "visits_per_item": a.visits_per_item},
+                   "tile_kernel": a.kernel, "visits_per_item": a.visits_per_item,
+                   "min_stage_run": a.min_stage_run, "item_group": a.item_group},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
@@ -350,6 +422,7 @@ def main():
         "accuracy": accuracy,
         "cpu_baseline": cpu,
         "dropin": dropin,
+        "weights": weights,
     }
     print(json.dumps(out), flush=True)
 

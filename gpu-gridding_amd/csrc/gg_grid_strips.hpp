@@ -193,8 +193,18 @@ void k_grid_strips(const GridArgs<float> A)
     if (threadIdx.x < 16) tab[kStripGuard - 16 + threadIdx.x] = make_float2(0.f, 0.f);
     int staged_base = -1, staged_orient = -1;      // table in LDS (uniform)
 
-    for (unsigned item = blockIdx.x; item < num_items; item += gridDim.x)
+    // Position p runs on XCD p % 8 (the launch is a multiple of 8
+    // workgroups, dealt out round-robin).  Within each block of 8g positions
+    // XCD x takes the g consecutive items x*g .. x*g+g-1 (g = item_group), so
+    // neighbouring regions -- which share the tables of the visibilities
+    // whose footprints straddle them -- run on one L2, while all XCDs still
+    // advance through the item list together.
+    const unsigned g = A.item_group, blk = 8u * g;
+    const unsigned npos = (num_items + blk - 1) / blk * blk;
+    for (unsigned p = blockIdx.x; p < npos; p += gridDim.x)
     {
+        const unsigned item = p / blk * blk + (p & 7u) * g + (p >> 3) % g;
+        if (item >= num_items) continue;
         const unsigned t = ld_const(A.item_tile, item);
         const unsigned ioff = ld_const(A.item_off, t);
         const unsigned m = item - ioff;

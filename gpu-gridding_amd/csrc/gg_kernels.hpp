@@ -790,6 +790,7 @@ struct GridArgs
     const typename V4<T>::t* vamp;
     unsigned vrec_bytes;           // unused
     unsigned run_bytes;            // run_start / run_z / run_tb allocations
+    unsigned item_group;           // strip kernel: consecutive items per XCD (>= 1)
 };
 
 // TwoSum (Knuth): s + e == a + b exactly.

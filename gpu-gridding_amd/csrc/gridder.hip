@@ -136,6 +136,7 @@ struct mi355_ctx
     bool cache_kernels = false;
     int grid_kernel = 0;            // 0 auto (strips for plain FP32), 1 direct 64x64, 2 staged 64x64
     unsigned min_stage_run = 6;
+    unsigned item_group = 8;        // strip kernel: consecutive work items per XCD
     int num_cu = 256;
 
     // Host-sync-free mode (buffers sized from upper bounds, counts read on
@@ -778,6 +779,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     ga.vmeta = C->vmeta.as<int4>();
     ga.vamp = C->vamp.as<T4>();
     ga.vrec_bytes = 0;   // (the strip kernel sizes its record windows per item)
+    ga.item_group = C->item_group;
     const unsigned padded = (unsigned)((num_items + 7) & ~7ull);
     // One workgroup per work item, in item order (consecutive items of a
     // heavy tile land on different XCDs); host-sync-free, a persistent grid
@@ -789,7 +791,11 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     {
         if constexpr (std::is_same<T, float>::value)
         {
-            hipLaunchKernelGGL(k_grid_strips, dim3(wg_items), dim3(kStripBlock), 0, C->stream, ga);
+            // (a multiple of 8 workgroups: the kernel maps positions to XCDs)
+            const unsigned long long blk = 8ull * C->item_group;
+            const unsigned wg8 = C->sync_free ? std::max(8u, wg_items & ~7u)
+                    : (unsigned)((num_items + blk - 1) / blk * blk);
+            hipLaunchKernelGGL(k_grid_strips, dim3(wg8), dim3(kStripBlock), 0, C->stream, ga);
             HIP_TRY(hipGetLastError());
         }
     }
@@ -1425,6 +1431,10 @@ MI355_EXPORT int mi355_ctx_set_option(mi355_ctx* C, int key, long long value)
         if (value < 1) return fail(MI355_ERR_INVALID_ARG, "min stage run %lld", value);
         C->min_stage_run = (unsigned)std::min<long long>(value, 0xFFFFFFFFll);
         break;
+    case MI355_OPT_ITEM_GROUP:
+        if (value < 1 || value > 4096) return fail(MI355_ERR_INVALID_ARG, "item group %lld", value);
+        C->item_group = (unsigned)value;
+        break;
     default: return fail(MI355_ERR_INVALID_ARG, "unknown option %d", key);
     }
     return ok();
@@ -1441,6 +1451,7 @@ MI355_EXPORT int mi355_ctx_get_option(const mi355_ctx* C, int key, long long* va
     case MI355_OPT_CACHE_KERNELS: *value = C->cache_kernels; break;
     case MI355_OPT_GRID_KERNEL: *value = C->grid_kernel; break;
     case MI355_OPT_MIN_STAGE_RUN: *value = C->min_stage_run; break;
+    case MI355_OPT_ITEM_GROUP: *value = C->item_group; break;
     default: return fail(MI355_ERR_INVALID_ARG, "unknown option %d", key);
     }
     return ok();
@@ -1688,23 +1699,25 @@ MI355_EXPORT int mi355_autotune_f_batched_dev(mi355_ctx* C, size_t P, const int*
         J[j].skipped = tskip.as<unsigned long long>() + j;
     }
     const bool timing0 = C->timing;
-    const unsigned vpi0 = C->visits_per_item, msr0 = C->min_stage_run;
+    const unsigned vpi0 = C->visits_per_item, msr0 = C->min_stage_run, ig0 = C->item_group;
     const int gk0 = C->grid_kernel;
     C->timing = true;
-    struct Cand { int kernel; unsigned vpi, msr; };
+    struct Cand { int kernel; unsigned vpi, msr, group; };
     std::vector<Cand> cands;
     for (int kern : {0, 2})
         for (unsigned vpi : {1024u, 2048u, 4096u, 8192u})
-            for (unsigned msr : {2u, 4u, 8u})
-                if (kern == 0 || msr == 4u) cands.push_back({kern, vpi, msr});
+            for (unsigned msr : {2u, 4u, 6u, 8u})
+                for (unsigned grp : {1u, 8u})
+                    if (kern == 0 || (msr == 4u && grp == ig0)) cands.push_back({kern, vpi, msr, grp});
     double best = 1e300;
-    Cand bc = {gk0, vpi0, msr0};
+    Cand bc = {gk0, vpi0, msr0, ig0};
     int st = MI355_OK;
     for (const Cand& c : cands)
     {
         C->grid_kernel = c.kernel;
         C->visits_per_item = c.vpi;
         C->min_stage_run = c.msr;
+        C->item_group = c.group;
         double t = 0;
         for (int rep = 0; rep < 2 && st == MI355_OK; ++rep)   // first run warms up
         {
@@ -1720,6 +1733,7 @@ MI355_EXPORT int mi355_autotune_f_batched_dev(mi355_ctx* C, size_t P, const int*
     C->grid_kernel = st == MI355_OK ? bc.kernel : gk0;
     C->visits_per_item = st == MI355_OK ? bc.vpi : vpi0;
     C->min_stage_run = st == MI355_OK ? bc.msr : msr0;
+    C->item_group = st == MI355_OK ? bc.group : ig0;
     release(tgrid); release(tnorm); release(tskip);
     if (st != MI355_OK) return st;
     if (best_ms) *best_ms = best;

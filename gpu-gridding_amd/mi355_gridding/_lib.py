@@ -27,6 +27,7 @@ OPT_PHASE_TIMING = 3
 OPT_CACHE_KERNELS = 4
 OPT_GRID_KERNEL = 5
 OPT_MIN_STAGE_RUN = 6
+OPT_ITEM_GROUP = 7
 
 
 class GridderError(RuntimeError):

@@ -64,7 +64,7 @@ def test_degrid_against_oracle_adjoint(gpu):
 
 
 def test_autograd_finite_differences_fp64(gpu):
-    """Config 4 (1M visibilities, 4096^2, 128 planes, support <= 32) in FP64:
+    """The differentiable workload (1M visibilities, 4096^2, 128 planes, support <= 32) in FP64:
     loss = 0.5 ||grid - target||^2 + 0.5 norm^2."""
     torch, ops = gpu
     dt = torch.float64
