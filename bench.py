@@ -20,7 +20,9 @@ FETCH_SIZE / WRITE_SIZE counter summary in profiles/, when present.
 from its sources into oracle/_ref, else the oracle's restatement) on one
 core on a prefix of the same visibilities.  `dropin` (with --dropin) times
 the host-pointer drop-in mi355_grid_wproj_f, host-device copies included.
-`weights` times the uniform-weighting pre-pass on the same points (device
+`image` times the grid -> image transform (centred inverse 2-D FFT of the
+gridded 16384^2 grid, real part / norm) beside numpy's FFT of a 4096^2
+grid on one core.  `weights` times the uniform-weighting pre-pass on the same points (device
 write of the 16384^2 weights grid, then the read of every point's weight),
 beside the reference's serial oskar_grid_weights_write_f / _read_f over all
 points on one core.
@@ -221,6 +223,45 @@ def bench_weights(case, d, ctx, no_cpu, reps=5):
     return out
 
 
+def bench_image(grid, G, norm, ctx, no_cpu, reps=5):
+    """Grid -> image on the device over the gridded grid; numpy's FFT of a
+    4096^2 grid on the host as the CPU figure."""
+    import numpy as np
+    import torch
+    from mi355_gridding import torch_ops as ops
+    img = torch.empty((G, G), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    ops.grid_to_image(grid, G, img, norm=norm, ctx=ctx)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        ops.grid_to_image(grid, G, img, norm=norm, ctx=ctx)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del img
+    gb = (8.0 * G * G + 4.0 * G * G) / 1e9       # grid read once, image written once
+    out = dict(ms=round(ms, 4), grid_size=G, value=round(G * G / (ms * 1e-3) / 1e9, 3),
+               unit="Gcells/s", min_bytes_gb=round(gb, 3),
+               eff_gbps=round(gb / (ms * 1e-3), 1),
+               note="hipFFT backward C2C in place on a work copy, plus the centring-sign "
+                    "and real-part/scale passes; eff_gbps counts only the grid read and the "
+                    "image write")
+    if not no_cpu:
+        n = 4096
+        rng = np.random.default_rng(0)
+        h = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))).astype(np.complex64)
+        t0 = time.perf_counter()
+        np.fft.ifft2(h)
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(value=round(n * n / t / 1e9, 4), unit="Gcells/s", cores=1,
+                                   kind="numpy", sample="numpy.fft.ifft2 of one %d^2 complex64 "
+                                   "grid (pocketfft, FP64 internally): %.2f s" % (n, t))
+    log("image %s" % json.dumps(out))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,6 +283,8 @@ def main():
                     help="also time the host-pointer drop-in (PCIe included)")
     ap.add_argument("--no-weights", action="store_true",
                     help="skip the uniform-weighting pre-pass leg")
+    ap.add_argument("--no-image", action="store_true",
+                    help="skip the grid -> image transform leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run plain device copies of MIB MiB, then exit")
     a = ap.parse_args()
@@ -377,6 +420,8 @@ def main():
         log("drop-in %s" % json.dumps(dropin))
 
     weights = None if a.no_weights else bench_weights(case, d, ctx, a.no_cpu_baseline)
+    image = None if a.no_image else bench_image(grid, G, float(norm.item()), ctx,
+                                                a.no_cpu_baseline)
 
     traffic = None
     kname = {"strips": "k_grid_strips", "staged": "k_grid_staged", "direct": "k_grid_tiles"}[a.kernel]
@@ -423,6 +468,7 @@ def main():
         "cpu_baseline": cpu,
         "dropin": dropin,
         "weights": weights,
+        "image": image,
     }
     print(json.dumps(out), flush=True)
 

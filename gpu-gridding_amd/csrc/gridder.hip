@@ -8,7 +8,10 @@
 #include "gg_grid_staged.hpp"
 #include "gg_grid_strips.hpp"
 #include "gg_weights.hpp"
+#include "gg_image.hpp"
 #include "mi355_gridder.h"
+
+#include <hipfft/hipfft.h>
 
 #include <algorithm>
 #include <cstdarg>
@@ -176,6 +179,11 @@ struct mi355_ctx
     // drop-in staging
     DevBuf in_conv, in_uu, in_vv, in_ww, in_vis, in_weight, in_grid, in_norm, in_skipped;
     DevBuf in_weight_out;
+    // grid -> image
+    DevBuf fft_work, in_image;
+    hipfftHandle fft_plan = 0;
+    bool fft_valid = false;
+    int fft_G = 0, fft_double = 0;
 
     // host mirrors kept alive across asynchronous uploads
     std::vector<unsigned char> h_jobs;
@@ -1320,6 +1328,82 @@ int weights_read_host(size_t N, const T* uu, const T* vv, const T* weight_in, T*
     return ok();
 }
 
+// ---------------------------------------------------------------------------
+// Grid -> image (centred inverse 2-D DFT, real part, scaled by 1/norm)
+// ---------------------------------------------------------------------------
+#define FFT_TRY(expr)                                                        \
+    do {                                                                     \
+        hipfftResult r_ = (expr);                                            \
+        if (r_ != HIPFFT_SUCCESS)                                            \
+            return fail(MI355_ERR_HIP, "%s failed: hipFFT status %d (%s:%d)", \
+                    #expr, (int)r_, __FILE__, __LINE__);                     \
+    } while (0)
+
+int check_image_args(int G)
+{
+    if (G < 2 || (G & 1)) return fail(MI355_ERR_INVALID_ARG, "grid_size must be even and >= 2");
+    if (G > 65536) return fail(MI355_ERR_TOO_LARGE, "grid_size > 65536 for the image transform");
+    return MI355_OK;
+}
+
+template <typename T>
+int image_dev(mi355_ctx* C, int G, const T* grid, double norm, T* image)
+{
+    TRY(check_image_args(G));
+    if (!grid || !image) return fail(MI355_ERR_INVALID_ARG, "NULL pointer argument");
+    HIP_TRY(hipSetDevice(C->device));
+    typedef typename V2<T>::t T2;
+    const unsigned long long n = (unsigned long long)G * G;
+    TRY(ensure(C->fft_work, n * sizeof(T2)));
+    const int dbl = sizeof(T) == 8;
+    if (!C->fft_valid || C->fft_G != G || C->fft_double != dbl)
+    {
+        if (C->fft_valid) (void)hipfftDestroy(C->fft_plan);
+        C->fft_valid = false;
+        FFT_TRY(hipfftPlan2d(&C->fft_plan, G, G, dbl ? HIPFFT_Z2Z : HIPFFT_C2C));
+        C->fft_valid = true;
+        C->fft_G = G;
+        C->fft_double = dbl;
+    }
+    FFT_TRY(hipfftSetStream(C->fft_plan, C->stream));
+    const unsigned blocks = grid_blocks(n / 2, 256, 65535);
+    hipLaunchKernelGGL(k_image_pre<T>, dim3(blocks), dim3(256), 0, C->stream,
+            reinterpret_cast<const T2*>(grid), C->fft_work.as<T2>(), (unsigned)G, n / 2);
+    HIP_TRY(hipGetLastError());
+    if (dbl)
+        FFT_TRY(hipfftExecZ2Z(C->fft_plan, C->fft_work.as<hipfftDoubleComplex>(),
+                C->fft_work.as<hipfftDoubleComplex>(), HIPFFT_BACKWARD));
+    else
+        FFT_TRY(hipfftExecC2C(C->fft_plan, C->fft_work.as<hipfftComplex>(),
+                C->fft_work.as<hipfftComplex>(), HIPFFT_BACKWARD));
+    const T scale = (T)(norm != 0.0 ? 1.0 / norm : 1.0);
+    hipLaunchKernelGGL(k_image_post<T>, dim3(blocks), dim3(256), 0, C->stream,
+            C->fft_work.as<T2>(), image, (unsigned)G, n / 2, scale);
+    HIP_TRY(hipGetLastError());
+    return ok();
+}
+
+template <typename T>
+int image_host(int G, const T* grid, double norm, T* image)
+{
+    TRY(check_image_args(G));
+    if (!grid || !image) return fail(MI355_ERR_INVALID_ARG, "NULL pointer argument");
+    std::lock_guard<std::mutex> lock(g_default_lock);
+    mi355_ctx* C = nullptr;
+    TRY(get_default_ctx(&C));
+    HIP_TRY(hipSetDevice(C->device));
+    const size_t n = (size_t)G * G;
+    TRY(ensure(C->in_grid, 2 * n * sizeof(T)));
+    TRY(ensure(C->in_image, n * sizeof(T)));
+    HIP_TRY(hipMemcpyAsync(C->in_grid.p, grid, 2 * n * sizeof(T), hipMemcpyHostToDevice,
+            C->stream));
+    TRY(image_dev<T>(C, G, C->in_grid.as<T>(), norm, C->in_image.as<T>()));
+    HIP_TRY(hipMemcpyAsync(image, C->in_image.p, n * sizeof(T), hipMemcpyDeviceToHost,
+            C->stream));
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    return ok();
+}
+
 } // namespace
 
 // ===========================================================================
@@ -1397,7 +1481,9 @@ MI355_EXPORT int mi355_ctx_destroy(mi355_ctx* C)
         &C->run_z, &C->run_tb, &C->scratch,
         &C->jobs_d, &C->job_chunk_off_d, &C->job_vis_off_d, &C->job_tile_base_d,
         &C->job_out_d, &C->job_bbox_d, &C->in_conv, &C->in_uu, &C->in_vv, &C->in_ww,
-        &C->in_vis, &C->in_weight, &C->in_grid, &C->in_norm, &C->in_skipped, &C->in_weight_out};
+        &C->in_vis, &C->in_weight, &C->in_grid, &C->in_norm, &C->in_skipped, &C->in_weight_out,
+        &C->fft_work, &C->in_image};
+    if (C->fft_valid) (void)hipfftDestroy(C->fft_plan);
     for (DevBuf* b : all) release(*b);
     for (int i = 0; i < EV_COUNT; ++i) if (C->ev[i]) (void)hipEventDestroy(C->ev[i]);
     delete C;
@@ -1807,6 +1893,33 @@ MI355_EXPORT int mi355_grid_weights_read_d_dev(mi355_ctx* C, size_t N, const dou
     if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
     return weights_read_dev<double>(C, N, uu, vv, weight_in, weight_out, cell, G, num_skipped,
             grid);
+}
+
+// ---- grid -> image -------------------------------------------------------
+MI355_EXPORT void mi355_grid_to_image_f(int grid_size, const float* grid, double norm,
+        float* image)
+{
+    (void)image_host<float>(grid_size, grid, norm, image);
+}
+
+MI355_EXPORT void mi355_grid_to_image_d(int grid_size, const double* grid, double norm,
+        double* image)
+{
+    (void)image_host<double>(grid_size, grid, norm, image);
+}
+
+MI355_EXPORT int mi355_grid_to_image_f_dev(mi355_ctx* C, int grid_size, const float* grid,
+        double norm, float* image)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    return image_dev<float>(C, grid_size, grid, norm, image);
+}
+
+MI355_EXPORT int mi355_grid_to_image_d_dev(mi355_ctx* C, int grid_size, const double* grid,
+        double norm, double* image)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    return image_dev<double>(C, grid_size, grid, norm, image);
 }
 
 } // extern "C"

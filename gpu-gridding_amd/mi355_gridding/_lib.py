@@ -111,6 +111,10 @@ SIGNATURES = {
     "mi355_grid_weights_write_d_dev": (I, [P, SZ, P, P, P, D, I, P, P]),
     "mi355_grid_weights_read_f_dev": (I, [P, SZ, P, P, P, P, F, I, P, P]),
     "mi355_grid_weights_read_d_dev": (I, [P, SZ, P, P, P, P, D, I, P, P]),
+    "mi355_grid_to_image_f": (None, [I, P, D, P]),
+    "mi355_grid_to_image_d": (None, [I, P, D, P]),
+    "mi355_grid_to_image_f_dev": (I, [P, I, P, D, P]),
+    "mi355_grid_to_image_d_dev": (I, [P, I, P, D, P]),
 }
 
 _lib = None

@@ -327,3 +327,22 @@ def grid_weights_read(uu, vv, weight_in, weight_out, cell_size_rad, grid_size, g
     _lib.check(fn(ctx.handle, n, _dp(uu), _dp(vv), _dp(weight_in), _dp(weight_out),
                   scal(float(cell_size_rad)), G, _dp(num_skipped), _dp(grid)))
     return weight_out
+
+
+def grid_to_image(grid, grid_size, image, norm=0.0, ctx=None):
+    """image (real [G, G]) = centred inverse 2-D DFT of grid (complex [G, G]
+    or real [G, G, 2]), real part, / norm (mi355_grid_to_image_*_dev; FP32 or
+    FP64 by the grid's precision).  norm may be a float or a 1-element
+    tensor (read on the host)."""
+    ctx = ctx or context()
+    G = int(grid_size)
+    g = torch.view_as_real(grid) if grid.is_complex() else grid
+    real = g.dtype
+    _check_dev(g, real, 2 * G * G, "grid")
+    _check_dev(image, real, G * G, "image")
+    if torch.is_tensor(norm):
+        norm = float(norm.item())
+    lib = _lib.load()
+    fn = lib.mi355_grid_to_image_f_dev if real == torch.float32 else lib.mi355_grid_to_image_d_dev
+    _lib.check(fn(ctx.handle, G, _dp(g), float(norm), _dp(image)))
+    return image

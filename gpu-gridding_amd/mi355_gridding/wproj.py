@@ -169,3 +169,26 @@ def grid_weights_read_d(num_points, uu, vv, weight_in, weight_out, cell_size_rad
     """Mirrors oskar_grid_weights_read_d (oskar_grid_weights.h:84-88)."""
     return _weights_read(_lib.load().mi355_grid_weights_read_d, np.float64, num_points, uu, vv,
                          weight_in, weight_out, cell_size_rad, grid_size, grid)
+
+
+def grid_to_image(grid, norm=0.0):
+    """Image of a gridded [G, G] complex grid (complex64 -> FP32 path,
+    complex128 -> FP64), mi355_grid_to_image_f / _d: the centred inverse 2-D
+    DFT, real part, divided by norm (1 when norm is 0).  Returns a real
+    [G, G] array."""
+    g = np.asarray(grid)
+    if g.dtype == np.complex64 or g.dtype == np.float32:
+        real, fn = np.float32, _lib.load().mi355_grid_to_image_f
+    elif g.dtype == np.complex128 or g.dtype == np.float64:
+        real, fn = np.float64, _lib.load().mi355_grid_to_image_d
+    else:
+        raise ValueError("grid must be complex64/complex128 (or float [G, G, 2])")
+    g = np.ascontiguousarray(g)
+    n = g.size if np.iscomplexobj(g) else g.size // 2
+    G = int(round(n ** 0.5))
+    if G * G != n:
+        raise ValueError("grid must be G x G complex")
+    image = np.empty((G, G), real)
+    fn(G, _ptr(g), float(norm), _ptr(image))
+    _lib.check_last()
+    return image

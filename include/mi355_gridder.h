@@ -307,6 +307,29 @@ MI355_EXPORT int mi355_grid_weights_read_d_dev(mi355_ctx* ctx, size_t num_points
         double cell_size_rad, int grid_size, unsigned long long* num_skipped,
         const double* grid);
 
+/* ------------------------------------------------------------------------
+ * Grid -> image: the 2-D FFT of the grid that README:24-26 names as the step
+ * after gridding (the reference has no code for it).  For an even
+ * grid_size G and c = G/2,
+ *   image[y][x] = Re( sum_{v,u} grid[v][u]
+ *                     * exp(+2 pi i ((u-c)(x-c) + (v-c)(y-c)) / G) ) / norm
+ * (1/norm taken as 1 when norm is 0): the centred inverse transform, so a
+ * grid holding one value at cell (c, c) gives a constant image.  grid is G*G
+ * complex values (interleaved re, im, row-major), image G*G reals.  hipFFT
+ * does the transform; two HIP passes apply the centring signs and the
+ * scale.  Odd grid sizes: MI355_ERR_INVALID_ARG; above 65536:
+ * MI355_ERR_TOO_LARGE.  The device forms use a work buffer of G*G complex
+ * values kept in the context.
+ * ---------------------------------------------------------------------- */
+MI355_EXPORT void mi355_grid_to_image_f(int grid_size, const float* grid, double norm,
+        float* image);
+MI355_EXPORT void mi355_grid_to_image_d(int grid_size, const double* grid, double norm,
+        double* image);
+MI355_EXPORT int mi355_grid_to_image_f_dev(mi355_ctx* ctx, int grid_size, const float* grid,
+        double norm, float* image);
+MI355_EXPORT int mi355_grid_to_image_d_dev(mi355_ctx* ctx, int grid_size, const double* grid,
+        double norm, double* image);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,13 @@
  * pinned against the same functions compiled from the reference (oracle/_ref)
  * and by closed forms (unit weights give per-cell counts; read after write
  * gives weight / cell total).
+ *
+ * Grid -> image (the "2D FFT of the grid" that README:24-26 names as the
+ * step after gridding; the reference has no code for it):
+ *   oracle_grid_to_image_d   centred inverse DFT, real part, scaled by
+ *                            1/norm, by direct summation in FP64
+ * pinned by closed forms (point sources at known cells give cosine images)
+ * and by numpy's FFT on the same grid.
  */
 #include <math.h>
 #include <stdint.h>
@@ -528,4 +535,63 @@ ORACLE_API void oracle_grid_weights_read_d(size_t n, const double* uu, const dou
         if (!weights_cell_d(uu[i], vv[i], grid_scale, grid_size, &t)) { ++*num_skipped; continue; }
         weight_out[i] = (grid[t] != 0.0) ? weight_in[i] / grid[t] : 0.0;
     }
+}
+
+/* ------------------------------------------------------------------------
+ * Grid -> image.  For an even grid size G, c = G/2:
+ *   image[y][x] = Re( sum_{v,u} grid[v][u]
+ *                     * exp(+2 pi i ((u - c)(x - c) + (v - c)(y - c)) / G) )
+ *                 * scale,   scale = 1/norm (1 when norm is 0).
+ * Separable direct summation (rows, then columns) in FP64, with the phase
+ * index ((u - c)(x - c)) mod G taken exactly in integers.
+ * grid: G*G complex values, interleaved (re, im), row v, column u.
+ * ---------------------------------------------------------------------- */
+ORACLE_API int oracle_grid_to_image_d(int G, const double* grid, double norm, double* image)
+{
+    if (G < 2 || (G & 1)) return 1;
+    const long long c = G / 2;
+    const double scale = norm != 0.0 ? 1.0 / norm : 1.0;
+    double* cs = (double*) malloc(2 * (size_t)G * sizeof(double));
+    double* tmp = (double*) malloc(2 * (size_t)G * G * sizeof(double));
+    if (!cs || !tmp) { free(cs); free(tmp); return 2; }
+    for (int k = 0; k < G; ++k)
+    {
+        const double a = 2.0 * M_PI * (double)k / (double)G;
+        cs[2 * k] = cos(a);
+        cs[2 * k + 1] = sin(a);
+    }
+    /* rows: tmp[v][x] = sum_u grid[v][u] e^{2 pi i (u-c)(x-c)/G} */
+    for (int v = 0; v < G; ++v)
+        for (int x = 0; x < G; ++x)
+        {
+            double re = 0.0, im = 0.0;
+            for (int u = 0; u < G; ++u)
+            {
+                long long k = ((long long)(u - c) * (long long)(x - c)) % G;
+                if (k < 0) k += G;
+                const double gr = grid[2 * ((size_t)v * G + u)];
+                const double gi = grid[2 * ((size_t)v * G + u) + 1];
+                re += gr * cs[2 * k] - gi * cs[2 * k + 1];
+                im += gr * cs[2 * k + 1] + gi * cs[2 * k];
+            }
+            tmp[2 * ((size_t)v * G + x)] = re;
+            tmp[2 * ((size_t)v * G + x) + 1] = im;
+        }
+    /* columns, real part only */
+    for (int y = 0; y < G; ++y)
+        for (int x = 0; x < G; ++x)
+        {
+            double re = 0.0;
+            for (int v = 0; v < G; ++v)
+            {
+                long long k = ((long long)(v - c) * (long long)(y - c)) % G;
+                if (k < 0) k += G;
+                re += tmp[2 * ((size_t)v * G + x)] * cs[2 * k] -
+                      tmp[2 * ((size_t)v * G + x) + 1] * cs[2 * k + 1];
+            }
+            image[(size_t)y * G + x] = re * scale;
+        }
+    free(cs);
+    free(tmp);
+    return 0;
 }

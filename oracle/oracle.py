@@ -62,6 +62,8 @@ def oracle():
                    lib.oracle_grid_wproj_fsel_d_tiles):
             fn.restype = None
         _weights_sigs(lib, "oracle_grid_weights_")
+        lib.oracle_grid_to_image_d.argtypes = [I, P, D, P]
+        lib.oracle_grid_to_image_d.restype = I
         _oracle = lib
     return _oracle
 
@@ -247,3 +249,19 @@ def weights_read(uu, vv, weight_in, weight_out, cell_size_rad, grid_size, grid, 
     fn(uu.size, _p(uu), _p(vv), _p(weight_in), _p(weight_out), scal(float(cell_size_rad)),
        int(grid_size), ctypes.byref(sk), _p(grid))
     return sk.value
+
+
+def grid_to_image(grid, norm=0.0):
+    """Centred inverse DFT of a [G, G] complex grid (any complex or [G, G, 2]
+    real layout), real part scaled by 1/norm, in FP64 by direct summation
+    (oracle_grid_to_image_d).  Returns float64 [G, G]."""
+    g = np.asarray(grid)
+    if np.iscomplexobj(g):
+        g = g.astype(np.complex128).view(np.float64)
+    g = np.ascontiguousarray(g, np.float64)
+    G = int(round((g.size // 2) ** 0.5))
+    assert g.size == 2 * G * G, "grid must be G x G complex"
+    out = np.empty((G, G), np.float64)
+    if oracle().oracle_grid_to_image_d(G, _p(g), float(norm), _p(out)):
+        raise ValueError("grid size must be even")
+    return out
