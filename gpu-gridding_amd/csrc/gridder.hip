@@ -9,6 +9,7 @@
 #include "gg_grid_strips.hpp"
 #include "gg_weights.hpp"
 #include "gg_image.hpp"
+#include "gg_status.hpp"
 #include "mi355_gridder.h"
 
 #include <hipfft/hipfft.h>
@@ -26,7 +27,7 @@ using namespace gg;
 // ---------------------------------------------------------------------------
 // Status
 // ---------------------------------------------------------------------------
-namespace {
+namespace ggs {
 
 thread_local int t_last_status = MI355_OK;
 thread_local char t_last_msg[512] = "";
@@ -47,6 +48,13 @@ int ok()
     t_last_msg[0] = 0;
     return MI355_OK;
 }
+
+} // namespace ggs
+
+namespace {
+
+using ggs::fail;
+using ggs::ok;
 
 #define HIP_TRY(expr)                                                        \
     do {                                                                     \
@@ -1411,8 +1419,8 @@ int image_host(int G, const T* grid, double norm, T* image)
 // ===========================================================================
 extern "C" {
 
-MI355_EXPORT int mi355_grid_last_error(void) { return t_last_status; }
-MI355_EXPORT const char* mi355_grid_last_error_message(void) { return t_last_msg; }
+MI355_EXPORT int mi355_grid_last_error(void) { return ggs::t_last_status; }
+MI355_EXPORT const char* mi355_grid_last_error_message(void) { return ggs::t_last_msg; }
 
 MI355_EXPORT const char* mi355_grid_error_string(int status)
 {
@@ -1425,6 +1433,8 @@ MI355_EXPORT const char* mi355_grid_error_string(int status)
     case MI355_ERR_HIP: return "HIP runtime error";
     case MI355_ERR_OUT_OF_MEMORY: return "device out of memory";
     case MI355_ERR_TOO_LARGE: return "problem exceeds 32-bit indexing";
+    case MI355_ERR_IO: return "file I/O error";
+    case MI355_ERR_FORMAT: return "file format error";
     default: return "unknown status";
     }
 }

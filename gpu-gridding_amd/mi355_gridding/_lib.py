@@ -1,4 +1,5 @@
-"""ctypes bindings of libmi355_gridder.so (the C ABI in include/mi355_gridder.h).
+"""ctypes bindings of libmi355_gridder.so (the C ABI in include/mi355_gridder.h
+and include/mi355_io.h).
 
 The library is loaded from the package's own ``lib/`` directory, where the
 build puts it; there is no fallback implementation, so a missing or stale
@@ -20,6 +21,11 @@ ERR_NO_DEVICE = 3
 ERR_HIP = 4
 ERR_OUT_OF_MEMORY = 5
 ERR_TOO_LARGE = 6
+ERR_IO = 7
+ERR_FORMAT = 8
+
+# OSKAR binary payload types (include/mi355_io.h).
+BIN_CHAR, BIN_INT, BIN_SINGLE, BIN_DOUBLE, BIN_COMPLEX, BIN_MATRIX = 1, 2, 4, 8, 0x20, 0x40
 
 OPT_VISITS_PER_ITEM = 1
 OPT_COMPENSATED = 2
@@ -71,6 +77,39 @@ class JobF(ctypes.Structure):
     ]
 
 
+class VisHeader(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "num_tags_per_block", "amp_type", "coord_precision", "max_times_per_block",
+        "num_times_total", "num_channels_total", "num_stations")] + [
+        ("freq_start_hz", ctypes.c_double), ("num_baselines", ctypes.c_int),
+        ("num_blocks", ctypes.c_int)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class KernelInfo(ctypes.Structure):
+    _fields_ = [("conv_size_half", ctypes.c_int), ("num_w_planes", ctypes.c_int),
+                ("oversample", ctypes.c_int), ("grid_size", ctypes.c_int),
+                ("cellsize_rad", ctypes.c_double), ("w_scale", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class DriverResult(ctypes.Structure):
+    _fields_ = [("num_blocks", ctypes.c_int), ("grid_size", ctypes.c_int),
+                ("num_w_planes", ctypes.c_int), ("precision", ctypes.c_int),
+                ("num_points", ctypes.c_ulonglong), ("num_skipped_total", ctypes.c_ulonglong),
+                ("num_skipped_last", ctypes.c_ulonglong), ("norm", ctypes.c_double),
+                ("seconds_read", ctypes.c_double), ("seconds_weights", ctypes.c_double),
+                ("seconds_grid", ctypes.c_double), ("check_max_rel", ctypes.c_double),
+                ("check_l2_rel", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 P = ctypes.c_void_p
 SZ = ctypes.c_size_t
 I = ctypes.c_int
@@ -115,6 +154,25 @@ SIGNATURES = {
     "mi355_grid_to_image_d": (None, [I, P, D, P]),
     "mi355_grid_to_image_f_dev": (I, [P, I, P, D, P]),
     "mi355_grid_to_image_d_dev": (I, [P, I, P, D, P]),
+    # mi355_io.h
+    "mi355_bin_open": (I, [ctypes.c_char_p, ctypes.c_char, ctypes.POINTER(P)]),
+    "mi355_bin_close": (I, [P]),
+    "mi355_bin_num_tags": (I, [P]),
+    "mi355_bin_set_query_search_start": (I, [P, I]),
+    "mi355_bin_query": (I, [P, ctypes.c_ubyte, ctypes.c_ubyte, ctypes.c_ubyte, I,
+                            ctypes.POINTER(SZ), ctypes.POINTER(I)]),
+    "mi355_bin_read": (I, [P, ctypes.c_ubyte, ctypes.c_ubyte, ctypes.c_ubyte, I, SZ, P]),
+    "mi355_bin_write": (I, [P, ctypes.c_ubyte, ctypes.c_ubyte, ctypes.c_ubyte, I, SZ, P]),
+    "mi355_vis_read_header": (I, [P, ctypes.POINTER(VisHeader)]),
+    "mi355_vis_read_block": (I, [P, ctypes.POINTER(VisHeader), I, SZ, ctypes.POINTER(SZ),
+                                 P, P, P, P]),
+    "mi355_fits_read_kernel": (I, [ctypes.c_char_p, I, ctypes.POINTER(KernelInfo),
+                                   ctypes.POINTER(P), ctypes.POINTER(P)]),
+    "mi355_fits_write_kernel": (I, [ctypes.c_char_p, I, ctypes.POINTER(KernelInfo), P, P]),
+    "mi355_fits_write_image": (I, [ctypes.c_char_p, I, I, I, P]),
+    "mi355_io_free": (None, [P]),
+    "mi355_grid_vis_file": (I, [P, ctypes.c_char_p, I, P, SZ, ctypes.c_char_p,
+                                ctypes.POINTER(DriverResult)]),
 }
 
 _lib = None

@@ -38,6 +38,8 @@ extern "C" {
 #define MI355_ERR_HIP              4  /* a HIP runtime call failed */
 #define MI355_ERR_OUT_OF_MEMORY    5  /* device allocation failed */
 #define MI355_ERR_TOO_LARGE        6  /* a count exceeds the library's 32-bit indexing */
+#define MI355_ERR_IO               7  /* file open / read / write failed (mi355_io.h) */
+#define MI355_ERR_FORMAT           8  /* file is not in the expected format (mi355_io.h) */
 
 /* ------------------------------------------------------------------------
  * Drop-in replacements (host pointers, synchronous).

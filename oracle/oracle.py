@@ -32,6 +32,7 @@ _W_READ_D = [SZ, P, P, P, P, D, I, P, P]
 
 _oracle = None
 _ref = None
+_ref_io = None
 
 
 def _weights_sigs(lib, prefix):
@@ -265,3 +266,54 @@ def grid_to_image(grid, norm=0.0):
     if oracle().oracle_grid_to_image_d(G, _p(g), float(norm), _p(out)):
         raise ValueError("grid size must be even")
     return out
+
+
+def ref_io():
+    """The reference's input-file code compiled from its sources
+    (oracle/_ref/libref_io.so: OSKAR binary library, cfitsio 3.41,
+    utils/read_vis.c and read_kernel.c), or None where it was not built."""
+    global _ref_io
+    if _ref_io is None:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_ref", "libref_io.so")
+        if not os.path.exists(path):
+            return None
+        lib = ctypes.CDLL(path)
+        UB, IP = ctypes.c_ubyte, ctypes.POINTER(I)
+        sigs = {
+            "oskar_binary_create": (P, [ctypes.c_char_p, ctypes.c_char, IP]),
+            "oskar_binary_free": (None, [P]),
+            "oskar_binary_num_tags": (I, [P]),
+            "oskar_binary_write": (None, [P, UB, UB, UB, I, SZ, P, IP]),
+            "oskar_binary_write_ext": (None, [P, UB, ctypes.c_char_p, ctypes.c_char_p, I, SZ,
+                                              P, IP]),
+            "oskar_binary_read": (None, [P, UB, UB, UB, I, SZ, P, IP]),
+            "oskar_binary_query": (I, [P, UB, UB, UB, I, ctypes.POINTER(SZ), IP]),
+            "oskar_binary_set_query_search_start": (None, [P, I, IP]),
+            "read_coords": (None, [P, I, I, SZ, P, P, P, IP]),
+            "scale_coords": (None, [D, I, SZ, P, P, P]),
+            "read_kernel": (P, [ctypes.c_char_p, I, IP, IP, ctypes.POINTER(P), IP, IP,
+                                ctypes.POINTER(D), ctypes.POINTER(D), IP]),
+            # cfitsio (fits_create_file, _create_img, _write_key, _write_pix,
+            # _create_tbl, _write_col, _close_file)
+            "ffinit": (I, [ctypes.POINTER(P), ctypes.c_char_p, IP]),
+            "ffcrim": (I, [P, I, I, ctypes.POINTER(ctypes.c_long), IP]),
+            "ffpky": (I, [P, I, ctypes.c_char_p, P, ctypes.c_char_p, IP]),
+            "ffppx": (I, [P, I, ctypes.POINTER(ctypes.c_long), ctypes.c_longlong, P, IP]),
+            "ffcrtb": (I, [P, I, ctypes.c_longlong, I, ctypes.POINTER(ctypes.c_char_p),
+                           ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
+                           ctypes.c_char_p, IP]),
+            "ffpcl": (I, [P, I, I, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, P,
+                          IP]),
+            "ffclos": (I, [P, IP]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _ref_io = lib
+    return _ref_io
+
+
+# cfitsio codes (fitsio.h:219-270)
+TSHORT, TINT, TFLOAT, TDOUBLE = 21, 31, 42, 82
+SHORT_IMG, FLOAT_IMG, DOUBLE_IMG, BINARY_TBL = 16, -32, -64, 2

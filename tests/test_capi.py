@@ -14,10 +14,11 @@ from conftest import ROOT
 from mi355_gridding import _lib, synth, wproj
 
 HEADER = os.path.join(ROOT, "include", "mi355_gridder.h")
+HEADERS = [HEADER, os.path.join(ROOT, "include", "mi355_io.h")]
 
 
 def declared_functions():
-    text = open(HEADER).read()
+    text = "".join(open(h).read() for h in HEADERS)
     return sorted(set(re.findall(r"MI355_EXPORT\s+[\w\s\*]+?\b(mi355_\w+)\s*\(", text)))
 
 
