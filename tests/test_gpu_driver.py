@@ -43,7 +43,7 @@ def make_dataset(root, prec, S=24, T=40, tpb=10, G=512, P=32, smax=12, ov=4, see
     th = th0[None, :] + np.linspace(0, 0.5, T)[:, None]      # tracks rotate with time
     uu, vv = r * np.cos(th), r * np.sin(th)
     ww = rng.normal(0, 1, (T, B)) * 0.3 * r
-    w_scale = (P - 1) ** 2 / np.percentile(np.abs(ww), 99.9) ** 2
+    w_scale = (P - 1) ** 2 / np.percentile(np.abs(ww), 99.9)     # planes span 0 .. P-1
     vis = (rng.normal(size=(T, B)) + 1j * rng.normal(size=(T, B))) / np.sqrt(2)
     kre = np.ascontiguousarray(conv[..., 0])
     kim = np.ascontiguousarray(conv[..., 1])
