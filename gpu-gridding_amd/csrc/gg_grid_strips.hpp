@@ -94,51 +94,41 @@ __device__ __forceinline__ X rb_x4(__amdgpu_buffer_rsrc_t rs, unsigned off)
 // Copy a table into LDS, mirrored as orient says, at stride kStripS: staged
 // row r + kStripPad, column c holds the tap a visit of this orientation reads
 // at footprint offset (r, c).  Staged rows 0 .. W+13 come from table rows
-// -7 .. W+6 of the padded block (the pad rows are zero); columns W ..
-// kStripS-1 are zero (out-of-range loads).  Wave w copies rows w, w+8, ...
-// kStageRowsInFlight at a time; lane l columns l, l+64 and l+128.
-constexpr int kStageRowsInFlight = 4;
+// -7 .. W+6 of the padded block (the pad rows are zero); columns W .. 127
+// are zero (out-of-range loads) and columns 128 .. kStripS-1 stay zero from
+// the kernel's start (no staged table reaches them).
+//
+// The copy goes straight from memory to LDS (buffer loads with the LDS
+// flag): no registers hold the data, so every row's loads are issued at
+// once and a table costs one round trip to memory.  Wave w copies rows w,
+// w + kStripWaves, ...; per row, four dword loads of the wave cover columns
+// 0 .. 127, lane l moving component l & 1 of column 32 k + l / 2, so the
+// mirrored column order is a per-lane source address.  The caller's barrier
+// after the copy waits for the loads (vmcnt) before any wave reads LDS.
+typedef __attribute__((address_space(3))) void lds_void_t;
+static_assert(kStripMaxW < 128, "staged columns 128 .. kStripS-1 are always zero");
 
 __device__ __forceinline__ void stage_strip_table(float2* tab, __amdgpu_buffer_rsrc_t rs,
         int tb, int W, int orient, int wave, int lane)
 {
     const int rows = W + 2 * kStripPad;
     const bool neg_v = orient & kFlagNegV, neg_u = orient & kFlagNegU;
-    unsigned vo[3];
+    unsigned vo[4];
     #pragma unroll
-    for (int h = 0; h < 3; ++h)
+    for (int k = 0; k < 4; ++k)
     {
-        const int c = lane + 64 * h;
-        vo[h] = c < W ? (unsigned)(neg_u ? W - 1 - c : c) * 8u : 0x80000000u;
+        const int c = 32 * k + (lane >> 1);
+        vo[k] = c < W ? (unsigned)((neg_u ? W - 1 - c : c) * 8 + (lane & 1) * 4) : 0x80000000u;
     }
-    float2* dst = tab + kStripGuard + lane;
-    for (int r0 = wave; r0 < rows; r0 += kStageRowsInFlight * kStripWaves)
+    for (int r = wave; r < rows; r += kStripWaves)
     {
-        float2 v[kStageRowsInFlight][3];
+        const int jr = r - kStripPad;
+        const unsigned so = (unsigned)((tb + (neg_v ? W - 1 - jr : jr) * W) * 8);
+        float2* row = tab + kStripGuard + r * kStripS;
         #pragma unroll
-        for (int u = 0; u < kStageRowsInFlight; ++u)
-        {
-            const int r = r0 + u * kStripWaves;
-            const int jr = r - kStripPad;
-            const unsigned so = (unsigned)((tb + (neg_v ? W - 1 - jr : jr) * W) * 8);
-            #pragma unroll
-            for (int h = 0; h < 3; ++h)
-            {
-                v[u][h] = make_float2(0.f, 0.f);
-                if (r < rows) v[u][h] = load_tap(rs, vo[h], so, (float2*)nullptr);
-            }
-        }
-        #pragma unroll
-        for (int u = 0; u < kStageRowsInFlight; ++u)
-        {
-            const int r = r0 + u * kStripWaves;
-            if (r < rows)
-            {
-                dst[r * kStripS] = v[u][0];
-                dst[r * kStripS + 64] = v[u][1];
-                if (lane + 128 < kStripS) dst[r * kStripS + 128] = v[u][2];
-            }
-        }
+        for (int k = 0; k < 4; ++k)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void_t*)(row + 32 * k), 4, vo[k],
+                    so, 0, 0);
     }
 }
 
@@ -190,7 +180,10 @@ void k_grid_strips(const GridArgs<float> A)
     // visit's footprint starts at strip row 0, column 0.
     const int lane_lds = kStripGuard + (kStripPad + lr) * kStripS + lc;
     const __amdgpu_buffer_rsrc_t rs_tab = table_rsrc(A.tables, A.table_bytes);
-    if (threadIdx.x < 16) tab[kStripGuard - 16 + threadIdx.x] = make_float2(0.f, 0.f);
+    // all of LDS zero: the 16 taps before staged row 0 and columns 128 ..
+    // kStripS-1 of every row are read as zeros and never written again
+    for (int i = threadIdx.x; i < kStripLds; i += kStripBlock) tab[i] = make_float2(0.f, 0.f);
+    __syncthreads();
     int staged_base = -1, staged_orient = -1;      // table in LDS (uniform)
 
     // Position p runs on XCD p % 8 (the launch is a multiple of 8
