@@ -91,9 +91,10 @@ __device__ __forceinline__ X rb_x4(__amdgpu_buffer_rsrc_t rs, unsigned off)
     return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0));
 }
 
-// Copy a table into LDS, mirrored as orient says, at stride kStripS: staged
-// row r + kStripPad, column c holds the tap a visit of this orientation reads
-// at footprint offset (r, c).  Staged rows 0 .. W+13 come from table rows
+// Copy a table into LDS, its rows mirrored when neg_v, at stride kStripS:
+// staged row r + kStripPad, column c holds the tap a visit with this row
+// orientation reads at footprint row r and table column c (visits mirrored
+// in u read columns backwards).  Staged rows 0 .. W+13 come from table rows
 // -7 .. W+6 of the padded block (the pad rows are zero); columns W .. 127
 // are zero (out-of-range loads) and columns 128 .. kStripS-1 stay zero from
 // the kernel's start (no staged table reaches them).
@@ -102,23 +103,22 @@ __device__ __forceinline__ X rb_x4(__amdgpu_buffer_rsrc_t rs, unsigned off)
 // flag): no registers hold the data, so every row's loads are issued at
 // once and a table costs one round trip to memory.  Wave w copies rows w,
 // w + kStripWaves, ...; per row, four dword loads of the wave cover columns
-// 0 .. 127, lane l moving component l & 1 of column 32 k + l / 2, so the
-// mirrored column order is a per-lane source address.  The caller's barrier
-// after the copy waits for the loads (vmcnt) before any wave reads LDS.
+// 0 .. 127, lane l moving component l & 1 of column 32 k + l / 2.  The
+// caller's barrier after the copy waits for the loads (vmcnt) before any
+// wave reads LDS.
 typedef __attribute__((address_space(3))) void lds_void_t;
 static_assert(kStripMaxW < 128, "staged columns 128 .. kStripS-1 are always zero");
 
 __device__ __forceinline__ void stage_strip_table(float2* tab, __amdgpu_buffer_rsrc_t rs,
-        int tb, int W, int orient, int wave, int lane)
+        int tb, int W, bool neg_v, int wave, int lane)
 {
     const int rows = W + 2 * kStripPad;
-    const bool neg_v = orient & kFlagNegV, neg_u = orient & kFlagNegU;
     unsigned vo[4];
     #pragma unroll
     for (int k = 0; k < 4; ++k)
     {
         const int c = 32 * k + (lane >> 1);
-        vo[k] = c < W ? (unsigned)((neg_u ? W - 1 - c : c) * 8 + (lane & 1) * 4) : 0x80000000u;
+        vo[k] = c < W ? (unsigned)(c * 8 + (lane & 1) * 4) : 0x80000000u;
     }
     for (int r = wave; r < rows; r += kStripWaves)
     {
@@ -176,9 +176,10 @@ void k_grid_strips(const GridArgs<float> A)
     const int lane = lane_id();
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lr = lane >> 4, lc = lane & 15;
-    // LDS tap of this lane for footprint offset (row lr, column lc) when the
-    // visit's footprint starts at strip row 0, column 0.
-    const int lane_lds = kStripGuard + (kStripPad + lr) * kStripS + lc;
+    // LDS row of this lane for footprint row lr when the visit's footprint
+    // starts at strip row 0; the column (lc, or backwards when mirrored in
+    // u) is added per visit.
+    const int lane_lds = kStripGuard + (kStripPad + lr) * kStripS;
     const __amdgpu_buffer_rsrc_t rs_tab = table_rsrc(A.tables, A.table_bytes);
     // all of LDS zero: the 16 taps before staged row 0 and columns 128 ..
     // kStripS-1 of every row are read as zeros and never written again
@@ -346,11 +347,12 @@ void k_grid_strips(const GridArgs<float> A)
                 const unsigned long long upto = starts ? (starts & (~starts + 1ull)) - 1ull : ~0ull;
                 const int rtb = bcast(mv.w, l0);
                 const int rz = bcast(mv.z, l0);
-                const int orient = rz & kOrientMask;
+                const int orient = rz & kFlagNegV;
                 if (rtb != staged_base || orient != staged_orient)
                 {
                     __syncthreads();
-                    stage_strip_table(tab, rs_tab, rtb, 2 * meta_support(rz) + 1, orient, wave, lane);
+                    stage_strip_table(tab, rs_tab, rtb, 2 * meta_support(rz) + 1, orient != 0,
+                            wave, lane);
                     __syncthreads();
                     staged_base = rtb;
                     staged_orient = orient;
@@ -361,7 +363,9 @@ void k_grid_strips(const GridArgs<float> A)
                     const int l = __builtin_ctzll(todo);
                     todo &= todo - 1ull;
                     GG_VISIT_HEAD
-                    const float2* t0 = tab + (lane_lds + kc - top * kStripS);
+                    // table column of the lane: kc + lc, or W-1-kc-lc mirrored
+                    const int lcol = (z & kFlagNegU) ? (2 * s - kc) - lc : kc + lc;
+                    const float2* t0 = tab + (lane_lds + lcol - top * kStripS);
                     const float2* t1 = t0 + 64 * kStripS;
                     GG_STRIP_PAIRS(cm)
                 }

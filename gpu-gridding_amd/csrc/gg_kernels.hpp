@@ -501,7 +501,7 @@ void k_emit(const int4* __restrict__ meta, const unsigned* __restrict__ num_visi
         const unsigned* __restrict__ visit_off, const JobDev<T>* __restrict__ jobs,
         const unsigned long long* __restrict__ job_vis_off, int num_jobs,
         unsigned long long n, int tw, int th, int key_table_shift, int key_table_bits,
-        unsigned* __restrict__ keys, unsigned* __restrict__ vals)
+        int orient_bits, unsigned* __restrict__ keys, unsigned* __restrict__ vals)
 {
     for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
             i < n; i += (unsigned long long)gridDim.x * blockDim.x)
@@ -518,9 +518,12 @@ void k_emit(const int4* __restrict__ meta, const unsigned* __restrict__ num_visi
         const unsigned tile_base = jobs[lo].tile_base;
         const int4 m = meta[i];
         const int s = meta_support(m.z);
-        // kernel table, then its orientation (the two mirror flags)
-        const unsigned kappa = ((meta_table_id(m.z) << 2) | (unsigned)(m.z & kOrientMask)) >>
-                key_table_shift;
+        // kernel table, then its orientation: both mirror flags
+        // (orient_bits = 2) or, for a kernel that mirrors columns when
+        // reading, only the row flag kFlagNegV (orient_bits = 1; kFlagNegV is
+        // bit 0 and kFlagNegU bit 1 of the packed word)
+        const unsigned kappa = ((meta_table_id(m.z) << orient_bits) |
+                (unsigned)(m.z & ((1 << orient_bits) - 1))) >> key_table_shift;
         const int tx0 = (m.x - s) / tw, tx1 = (m.x + s) / tw;
         const int ty0 = (m.y - s) / th, ty1 = (m.y + s) / th;
         unsigned o = visit_off[i];
@@ -658,12 +661,13 @@ __global__ void k_tile_bounds(const unsigned* __restrict__ keys,
 }
 
 // Runs: maximal stretches of sorted visits with the same tile, the same
-// kernel table and the same orientation.  flag[p] = 1 where a run starts.
-// When the sort key holds the full table id and orientation, key equality
-// decides; otherwise the visit records are compared.
+// kernel table and the same mirror flags of orient_mask.
+// flag[p] = 1 where a run starts.  When the sort key holds the full table id
+// (and orientation), key equality decides; otherwise the visit records are
+// compared.
 __global__ void k_run_flags(const unsigned* __restrict__ keys,
         const unsigned* __restrict__ vals, const int4* __restrict__ meta,
-        const unsigned long long* __restrict__ d_n, int exact_key,
+        const unsigned long long* __restrict__ d_n, int exact_key, int orient_mask,
         unsigned* __restrict__ flag)
 {
     const unsigned long long n = *d_n;
@@ -674,7 +678,7 @@ __global__ void k_run_flags(const unsigned* __restrict__ keys,
         if (!f && !exact_key)
         {
             const int4 a = meta[vals[p]], b = meta[vals[p - 1]];
-            f = a.w != b.w || ((a.z ^ b.z) & kOrientMask);
+            f = a.w != b.w || ((a.z ^ b.z) & orient_mask);
         }
         flag[p] = f;
     }

@@ -542,10 +542,14 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     if (tile_bits > 32) return fail(MI355_ERR_TOO_LARGE, "tile id exceeds 32 bits");
     const unsigned long long num_tables = (unsigned long long)P * noff * noff;
     // Sort key: tile, then the table id as the visit records hold it, then
-    // the two orientation bits; low bits are dropped if it exceeds 32 bits.
+    // the orientation bits: both mirror flags for the 64x64 staged kernels,
+    // which stage a table already mirrored, only the row flag for the strip
+    // kernel, which stages rows mirrored and mirrors columns when reading;
+    // low bits are dropped if the key exceeds 32 bits.
+    const int orient_bits = strips ? 1 : 2;
     const int table_bits_full = bits_for(num_tables);
     const int table_id_shift = std::max(0, table_bits_full - kTableIdBits);
-    const int kappa_bits = table_bits_full - table_id_shift + 2;
+    const int kappa_bits = table_bits_full - table_id_shift + orient_bits;
     const int key_table_bits = std::min(kappa_bits, 32 - tile_bits);
     const int key_table_shift = kappa_bits - key_table_bits;
 
@@ -670,7 +674,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     hipLaunchKernelGGL(k_emit<T>, dim3(grid_blocks(n_total, 256, 16384)), dim3(256), 0, C->stream,
             C->meta.as<int4>(), C->num_visits.as<unsigned>(), C->visit_off.as<unsigned>(),
             C->jobs_d.as<JobDev<T>>(), C->job_vis_off_d.as<unsigned long long>(), num_jobs,
-            n_total, tw, th, key_table_shift, key_table_bits, C->keys[0].as<unsigned>(),
+            n_total, tw, th, key_table_shift, key_table_bits, orient_bits, C->keys[0].as<unsigned>(),
             C->vals[0].as<unsigned>());
     HIP_TRY(hipGetLastError());
     const int key_bits = tile_bits + key_table_bits;
@@ -690,7 +694,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
         const int exact_key = table_id_shift == 0 && key_table_shift == 0;
         const unsigned vb = grid_blocks(T_vis, 256, 65535);
         hipLaunchKernelGGL(k_run_flags, dim3(vb), dim3(256), 0, C->stream, skeys, svals,
-                C->meta.as<int4>(), sc + SC_VISITS, exact_key, run_flag);
+                C->meta.as<int4>(), sc + SC_VISITS, exact_key, (1 << orient_bits) - 1, run_flag);
         DevCount rc{sc + SC_VISITS, 1, 1, 0};
         TRY(scan_u32(C, run_flag, run_excl, rc, T_vis, sc + SC_RUNS,
                 reinterpret_cast<unsigned*>(sc + SC_OVERFLOW), 0));
