@@ -91,13 +91,12 @@ __device__ __forceinline__ X rb_x4(__amdgpu_buffer_rsrc_t rs, unsigned off)
     return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0));
 }
 
-// Copy a table into LDS, its rows mirrored when neg_v, at stride kStripS:
-// staged row r + kStripPad, column c holds the tap a visit with this row
-// orientation reads at footprint row r and table column c (visits mirrored
-// in u read columns backwards).  Staged rows 0 .. W+13 come from table rows
-// -7 .. W+6 of the padded block (the pad rows are zero); columns W .. 127
-// are zero (out-of-range loads) and columns 128 .. kStripS-1 stay zero from
-// the kernel's start (no staged table reaches them).
+// Copy a table into LDS at stride kStripS: staged row t + kStripPad,
+// column c holds table row t, column c, for t = -7 .. W+6 (the pad rows of
+// the padded block are zero); columns W .. 127 are zero (out-of-range
+// loads) and columns 128 .. kStripS-1 stay zero from the kernel's start (no
+// staged table reaches them).  Visits of all four orientations read this one
+// copy: mirrored columns and rows are in their LDS addresses.
 //
 // The copy goes straight from memory to LDS (buffer loads with the LDS
 // flag): no registers hold the data, so every row's loads are issued at
@@ -110,7 +109,7 @@ typedef __attribute__((address_space(3))) void lds_void_t;
 static_assert(kStripMaxW < 128, "staged columns 128 .. kStripS-1 are always zero");
 
 __device__ __forceinline__ void stage_strip_table(float2* tab, __amdgpu_buffer_rsrc_t rs,
-        int tb, int W, bool neg_v, int wave, int lane)
+        int tb, int W, int wave, int lane)
 {
     const int rows = W + 2 * kStripPad;
     unsigned vo[4];
@@ -122,8 +121,7 @@ __device__ __forceinline__ void stage_strip_table(float2* tab, __amdgpu_buffer_r
     }
     for (int r = wave; r < rows; r += kStripWaves)
     {
-        const int jr = r - kStripPad;
-        const unsigned so = (unsigned)((tb + (neg_v ? W - 1 - jr : jr) * W) * 8);
+        const unsigned so = (unsigned)((tb + (r - kStripPad) * W) * 8);
         float2* row = tab + kStripGuard + r * kStripS;
         #pragma unroll
         for (int k = 0; k < 4; ++k)
@@ -168,6 +166,36 @@ __device__ __forceinline__ void mac_strip(float& re, float& im, const float2 c, 
         }                                                                     \
     }
 
+// The same for a visit whose table rows run backwards (off_v < 0): slot i
+// lies 4i rows further up the staged table, so a block's four slots are
+// read from its lowest address (that of its last slot) and assigned in
+// reverse.  t0 / t1: the lowest addresses of blocks 3 and 7 (slots 15 and
+// 31), so that every offset is positive.
+#define GG_STRIP_PAIRS_VMIRROR(cm)                                             \
+    {                                                                         \
+        _Pragma("unroll")                                                     \
+        for (int pq = 0; pq < kStripChunks / 2; ++pq)                         \
+        {                                                                     \
+            if (((cm) >> (2 * pq)) & 3)                                       \
+            {                                                                 \
+                const float2* pp = pq < 4 ? t0 + (3 - pq) * 16 * kStripS      \
+                                          : t1 + (7 - pq) * 16 * kStripS;     \
+                const float2 c0 = pp[0], c1 = pp[4 * kStripS];                \
+                const float2 c2 = pp[8 * kStripS], c3 = pp[12 * kStripS];     \
+                if ((cm) & (1 << (2 * pq)))                                   \
+                {                                                             \
+                    mac_strip(re[4 * pq], im[4 * pq], c3, a);                 \
+                    mac_strip(re[4 * pq + 1], im[4 * pq + 1], c2, a);         \
+                }                                                             \
+                if ((cm) & (2 << (2 * pq)))                                   \
+                {                                                             \
+                    mac_strip(re[4 * pq + 2], im[4 * pq + 2], c1, a);         \
+                    mac_strip(re[4 * pq + 3], im[4 * pq + 3], c0, a);         \
+                }                                                             \
+            }                                                                 \
+        }                                                                     \
+    }
+
 __global__ __launch_bounds__(kStripBlock) __attribute__((amdgpu_waves_per_eu(kStripWavesPerSimd)))
 void k_grid_strips(const GridArgs<float> A)
 {
@@ -176,16 +204,18 @@ void k_grid_strips(const GridArgs<float> A)
     const int lane = lane_id();
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lr = lane >> 4, lc = lane & 15;
-    // LDS row of this lane for footprint row lr when the visit's footprint
-    // starts at strip row 0; the column (lc, or backwards when mirrored in
-    // u) is added per visit.
+    // LDS row of this lane's table row when the visit's footprint starts at
+    // strip row 0: staged row kStripPad + lr, or kStripPad - lr (plus the
+    // visit's offset) when its rows run backwards; the column (lc, or
+    // backwards when mirrored in u) is added per visit.
     const int lane_lds = kStripGuard + (kStripPad + lr) * kStripS;
+    const int lane_lds_m = kStripGuard + (kStripPad - lr) * kStripS;
     const __amdgpu_buffer_rsrc_t rs_tab = table_rsrc(A.tables, A.table_bytes);
     // all of LDS zero: the 16 taps before staged row 0 and columns 128 ..
     // kStripS-1 of every row are read as zeros and never written again
     for (int i = threadIdx.x; i < kStripLds; i += kStripBlock) tab[i] = make_float2(0.f, 0.f);
     __syncthreads();
-    int staged_base = -1, staged_orient = -1;      // table in LDS (uniform)
+    int staged_base = -1;                          // table in LDS (uniform)
 
     // Position p runs on XCD p % 8 (the launch is a multiple of 8
     // workgroups, dealt out round-robin).  Within each block of 8g positions
@@ -340,6 +370,7 @@ void k_grid_strips(const GridArgs<float> A)
                                                       mv.x + sv >= xs);
             unsigned long long starts = __ballot(stg && (mv.z & kVisitRunStart)) |
                     (stg_mask & (~stg_mask + 1ull));
+            const unsigned long long vneg = __ballot(mv.z & kFlagNegV);
             while (starts)
             {
                 const int l0 = __builtin_ctzll(starts);
@@ -347,17 +378,16 @@ void k_grid_strips(const GridArgs<float> A)
                 const unsigned long long upto = starts ? (starts & (~starts + 1ull)) - 1ull : ~0ull;
                 const int rtb = bcast(mv.w, l0);
                 const int rz = bcast(mv.z, l0);
-                const int orient = rz & kFlagNegV;
-                if (rtb != staged_base || orient != staged_orient)
+                if (rtb != staged_base)
                 {
                     __syncthreads();
-                    stage_strip_table(tab, rs_tab, rtb, 2 * meta_support(rz) + 1, orient != 0,
-                            wave, lane);
+                    stage_strip_table(tab, rs_tab, rtb, 2 * meta_support(rz) + 1, wave, lane);
                     __syncthreads();
                     staged_base = rtb;
-                    staged_orient = orient;
                 }
-                unsigned long long todo = meets & upto & ~((1ull << l0) - 1ull);
+                const unsigned long long seg = meets & upto & ~((1ull << l0) - 1ull);
+                // visits whose rows run forwards, then those running backwards
+                unsigned long long todo = seg & ~vneg;
                 while (todo)
                 {
                     const int l = __builtin_ctzll(todo);
@@ -368,6 +398,18 @@ void k_grid_strips(const GridArgs<float> A)
                     const float2* t0 = tab + (lane_lds + lcol - top * kStripS);
                     const float2* t1 = t0 + 64 * kStripS;
                     GG_STRIP_PAIRS(cm)
+                }
+                todo = seg & vneg;
+                while (todo)
+                {
+                    const int l = __builtin_ctzll(todo);
+                    todo &= todo - 1ull;
+                    GG_VISIT_HEAD
+                    // table row of strip row 0: W-1+top, slot i 4i rows up
+                    const int lcol = (z & kFlagNegU) ? (2 * s - kc) - lc : kc + lc;
+                    const float2* t0 = tab + (lane_lds_m + lcol + (2 * s + top - 60) * kStripS);
+                    const float2* t1 = t0 - 64 * kStripS;
+                    GG_STRIP_PAIRS_VMIRROR(cm)
                 }
             }
         }
@@ -415,5 +457,6 @@ void k_grid_strips(const GridArgs<float> A)
 }
 
 #undef GG_STRIP_PAIRS
+#undef GG_STRIP_PAIRS_VMIRROR
 
 } // namespace gg

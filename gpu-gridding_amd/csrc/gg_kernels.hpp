@@ -518,12 +518,11 @@ void k_emit(const int4* __restrict__ meta, const unsigned* __restrict__ num_visi
         const unsigned tile_base = jobs[lo].tile_base;
         const int4 m = meta[i];
         const int s = meta_support(m.z);
-        // kernel table, then its orientation: both mirror flags
-        // (orient_bits = 2) or, for a kernel that mirrors columns when
-        // reading, only the row flag kFlagNegV (orient_bits = 1; kFlagNegV is
-        // bit 0 and kFlagNegU bit 1 of the packed word)
+        // kernel table, then (orient_bits = 2) its orientation, the two mirror
+        // flags; a kernel that mirrors when reading takes runs of all
+        // orientations (orient_bits = 0)
         const unsigned kappa = ((meta_table_id(m.z) << orient_bits) |
-                (unsigned)(m.z & ((1 << orient_bits) - 1))) >> key_table_shift;
+                (orient_bits ? (unsigned)(m.z & kOrientMask) : 0u)) >> key_table_shift;
         const int tx0 = (m.x - s) / tw, tx1 = (m.x + s) / tw;
         const int ty0 = (m.y - s) / th, ty1 = (m.y + s) / th;
         unsigned o = visit_off[i];

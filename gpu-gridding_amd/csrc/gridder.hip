@@ -542,11 +542,11 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     if (tile_bits > 32) return fail(MI355_ERR_TOO_LARGE, "tile id exceeds 32 bits");
     const unsigned long long num_tables = (unsigned long long)P * noff * noff;
     // Sort key: tile, then the table id as the visit records hold it, then
-    // the orientation bits: both mirror flags for the 64x64 staged kernels,
-    // which stage a table already mirrored, only the row flag for the strip
-    // kernel, which stages rows mirrored and mirrors columns when reading;
-    // low bits are dropped if the key exceeds 32 bits.
-    const int orient_bits = strips ? 1 : 2;
+    // (64x64 staged kernels, which stage a table already mirrored) the two
+    // orientation bits; the strip kernel mirrors when reading LDS, so its
+    // runs span orientations.  Low bits are dropped if the key exceeds 32
+    // bits.
+    const int orient_bits = strips ? 0 : 2;
     const int table_bits_full = bits_for(num_tables);
     const int table_id_shift = std::max(0, table_bits_full - kTableIdBits);
     const int kappa_bits = table_bits_full - table_id_shift + orient_bits;
@@ -694,7 +694,8 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
         const int exact_key = table_id_shift == 0 && key_table_shift == 0;
         const unsigned vb = grid_blocks(T_vis, 256, 65535);
         hipLaunchKernelGGL(k_run_flags, dim3(vb), dim3(256), 0, C->stream, skeys, svals,
-                C->meta.as<int4>(), sc + SC_VISITS, exact_key, (1 << orient_bits) - 1, run_flag);
+                C->meta.as<int4>(), sc + SC_VISITS, exact_key, orient_bits ? kOrientMask : 0,
+                run_flag);
         DevCount rc{sc + SC_VISITS, 1, 1, 0};
         TRY(scan_u32(C, run_flag, run_excl, rc, T_vis, sc + SC_RUNS,
                 reinterpret_cast<unsigned*>(sc + SC_OVERFLOW), 0));
