@@ -260,14 +260,14 @@ void k_grid_strips(const GridArgs<float> A)
         // (resources based at the item's records keep buffer offsets small;
         // reads past the item's last visit return zero)
         const __amdgpu_buffer_rsrc_t rs_vmeta = table_rsrc(A.vmeta + v0, (v1 - v0) * 16u);
-        const __amdgpu_buffer_rsrc_t rs_vamp = table_rsrc(A.vamp + v0, (v1 - v0) * 16u);
+        const __amdgpu_buffer_rsrc_t rs_vamp = table_rsrc(A.vamp + v0, (v1 - v0) * 8u);
 
         // The visits of a batch of 64 (lane l: visit b + l) that this strip
         // processes: those that meet its columns, of the wanted kind.
-        auto batch_mask = [&](unsigned b, unsigned e, int4& mv, float4& av, bool staged_kind) {
+        auto batch_mask = [&](unsigned b, unsigned e, int4& mv, float2& av, bool staged_kind) {
             const unsigned pl = b + (unsigned)lane;
             mv = rb_x4<int4>(rs_vmeta, (pl - v0) * 16u);
-            av = rb_x4<float4>(rs_vamp, (pl - v0) * 16u);
+            av = load_tap(rs_vamp, (pl - v0) * 8u, 0u, (float2*)nullptr);
             const int s = meta_support(mv.z);
             return __ballot(pl < e && ((mv.z & kVisitStaged) != 0) == staged_kind &&
                             mv.x - s <= xs + kStripW - 1 && mv.x + s >= xs);
@@ -276,11 +276,17 @@ void k_grid_strips(const GridArgs<float> A)
         // origin relative to the strip (kc: footprint column of strip column
         // 0; top: footprint row 0), chunks q0..q1 touched, coefficients.  (A
         // visit of the region's list meets its rows: top <= 127, top+2s >= 0.)
+// The FMA coefficients (re += c.x a.x + c.y a.y, im += c.x a.z + c.y a.w)
+// come from the weighted amplitude (vr, vi) and the conjugation flag:
+// (vr, -vi, vi, vr), or (vr, vi, vi, -vr) when conjugated.
 #define GG_VISIT_HEAD                                                           \
         const int x = bcast(mv.x, l), y = bcast(mv.y, l), z = bcast(mv.z, l);   \
         float4 a;                                                               \
-        a.x = bcast(av.x, l); a.y = bcast(av.y, l);                             \
-        a.z = bcast(av.z, l); a.w = bcast(av.w, l);                             \
+        {                                                                       \
+            const float vr = bcast(av.x, l), vi = bcast(av.y, l);               \
+            const bool cj = z & kFlagConj;                                      \
+            a.x = vr; a.y = cj ? vi : -vi; a.z = vi; a.w = cj ? -vr : vr;       \
+        }                                                                       \
         const int s = meta_support(z);                                          \
         const int kc = xs - (x - s);                                            \
         const int top = y - s - ys;                                             \
@@ -296,7 +302,7 @@ void k_grid_strips(const GridArgs<float> A)
         {
             __syncthreads();
             int4 mv;
-            float4 av;
+            float2 av;
             unsigned long long todo = batch_mask(b, v1, mv, av, false);
             while (todo)
             {
@@ -361,7 +367,7 @@ void k_grid_strips(const GridArgs<float> A)
         {
             const unsigned pl = b + (unsigned)lane;
             const int4 mv = rb_x4<int4>(rs_vmeta, (pl - v0) * 16u);
-            const float4 av = rb_x4<float4>(rs_vamp, (pl - v0) * 16u);
+            const float2 av = load_tap(rs_vamp, (pl - v0) * 8u, 0u, (float2*)nullptr);
             const bool stg = pl < v1 && (mv.z & kVisitStaged);
             const unsigned long long stg_mask = __ballot(stg);
             if (!stg_mask) continue;
@@ -395,8 +401,12 @@ void k_grid_strips(const GridArgs<float> A)
                     GG_VISIT_HEAD
                     // table column of the lane: kc + lc, or W-1-kc-lc mirrored
                     const int lcol = (z & kFlagNegU) ? (2 * s - kc) - lc : kc + lc;
-                    const float2* t0 = tab + (lane_lds + lcol - top * kStripS);
-                    const float2* t1 = t0 + 64 * kStripS;
+                    // (two independent bases: every block's reads then fit
+                    // the positive offset fields of ds_read2st64)
+                    int o0 = lane_lds + lcol - top * kStripS, o1 = o0 + 64 * kStripS;
+                    __asm__ volatile("" : "+v"(o0), "+v"(o1));
+                    const float2* t0 = tab + o0;
+                    const float2* t1 = tab + o1;
                     GG_STRIP_PAIRS(cm)
                 }
                 todo = seg & vneg;
@@ -407,8 +417,11 @@ void k_grid_strips(const GridArgs<float> A)
                     GG_VISIT_HEAD
                     // table row of strip row 0: W-1+top, slot i 4i rows up
                     const int lcol = (z & kFlagNegU) ? (2 * s - kc) - lc : kc + lc;
-                    const float2* t0 = tab + (lane_lds_m + lcol + (2 * s + top - 60) * kStripS);
-                    const float2* t1 = t0 - 64 * kStripS;
+                    int o0 = lane_lds_m + lcol + (2 * s + top - 60) * kStripS,
+                        o1 = o0 - 64 * kStripS;
+                    __asm__ volatile("" : "+v"(o0), "+v"(o1));
+                    const float2* t0 = tab + o0;
+                    const float2* t1 = tab + o1;
                     GG_STRIP_PAIRS_VMIRROR(cm)
                 }
             }

@@ -540,29 +540,32 @@ constexpr int kSortThreads = 256;
 constexpr int kSortPerThread = 16;
 constexpr int kSortChunk = kSortThreads * kSortPerThread;
 
-// Histogram of one 8-bit digit per chunk, stored digit-major:
+// Histogram of one BITS-bit digit (8 or 9) per chunk, stored digit-major:
 // hist[d * nchunk + c].
+template <int BITS>
 __global__ __launch_bounds__(kSortThreads)
 void k_sort_hist(const unsigned* __restrict__ keys,
         const unsigned long long* __restrict__ d_n, int shift,
         unsigned* __restrict__ hist)
 {
-    __shared__ unsigned h[256];
+    constexpr int R = 1 << BITS;
+    constexpr unsigned M = R - 1;
+    __shared__ unsigned h[R];
     const unsigned long long n = *d_n;
     const unsigned long long nchunk = (n + kSortChunk - 1) / kSortChunk;
     for (unsigned long long c = blockIdx.x; c < nchunk; c += gridDim.x)
     {
-        h[threadIdx.x] = 0;
+        for (int d = threadIdx.x; d < R; d += kSortThreads) h[d] = 0;
         __syncthreads();
         const unsigned long long base = c * kSortChunk;
         #pragma unroll 4
         for (int r = 0; r < kSortPerThread; ++r)
         {
             const unsigned long long idx = base + r * kSortThreads + threadIdx.x;
-            if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+            if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & M], 1u);
         }
         __syncthreads();
-        hist[threadIdx.x * nchunk + c] = h[threadIdx.x];
+        for (int d = threadIdx.x; d < R; d += kSortThreads) hist[d * nchunk + c] = h[d];
         __syncthreads();
     }
 }
@@ -570,6 +573,7 @@ void k_sort_hist(const unsigned* __restrict__ keys,
 // Stable scatter: within a chunk, elements are ranked in index order per
 // digit (wave-level match by ballots, then a prefix over the four waves),
 // and the chunk's base for each digit comes from the digit-major scan.
+template <int BITS>
 __global__ __launch_bounds__(kSortThreads)
 void k_sort_scatter(const unsigned* __restrict__ keys_in,
         const unsigned* __restrict__ vals_in,
@@ -577,20 +581,25 @@ void k_sort_scatter(const unsigned* __restrict__ keys_in,
         const unsigned* __restrict__ hist_off,
         unsigned* __restrict__ keys_out, unsigned* __restrict__ vals_out)
 {
-    __shared__ unsigned goff[256];
-    __shared__ unsigned run[256];
-    __shared__ unsigned wcnt[kSortThreads / 64][256];
-    __shared__ unsigned wbase[kSortThreads / 64][256];
+    constexpr int R = 1 << BITS;
+    constexpr unsigned M = R - 1;
+    __shared__ unsigned goff[R];
+    __shared__ unsigned run[R];
+    __shared__ unsigned wcnt[kSortThreads / 64][R];
+    __shared__ unsigned wbase[kSortThreads / 64][R];
     const unsigned long long n = *d_n;
     const unsigned long long nchunk = (n + kSortChunk - 1) / kSortChunk;
-    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    const int wave = threadIdx.x >> 6;
     const unsigned long long lt = lanemask_lt();
     for (unsigned long long c = blockIdx.x; c < nchunk; c += gridDim.x)
     {
-        goff[threadIdx.x] = hist_off[threadIdx.x * nchunk + c];
-        run[threadIdx.x] = 0;
-        #pragma unroll
-        for (int w = 0; w < kSortThreads / 64; ++w) wcnt[w][threadIdx.x] = 0;
+        for (int d = threadIdx.x; d < R; d += kSortThreads)
+        {
+            goff[d] = hist_off[d * nchunk + c];
+            run[d] = 0;
+            #pragma unroll
+            for (int w = 0; w < kSortThreads / 64; ++w) wcnt[w][d] = 0;
+        }
         __syncthreads();
         const unsigned long long base = c * kSortChunk;
         for (int r = 0; r < kSortPerThread; ++r)
@@ -599,10 +608,10 @@ void k_sort_scatter(const unsigned* __restrict__ keys_in,
             const bool valid = idx < n;
             const unsigned key = valid ? keys_in[idx] : 0u;
             const unsigned val = valid ? vals_in[idx] : 0u;
-            const unsigned d = (key >> shift) & 255u;
+            const unsigned d = (key >> shift) & M;
             unsigned long long peers = __ballot(valid);
             #pragma unroll
-            for (int b = 0; b < 8; ++b)
+            for (int b = 0; b < BITS; ++b)
             {
                 const bool bit = (d >> b) & 1u;
                 const unsigned long long bal = __ballot(bit);
@@ -611,16 +620,17 @@ void k_sort_scatter(const unsigned* __restrict__ keys_in,
             const unsigned rank = (unsigned)__popcll(peers & lt);
             if (valid && rank == 0) wcnt[wave][d] = (unsigned)__popcll(peers);
             __syncthreads();
+            for (int e = threadIdx.x; e < R; e += kSortThreads)
             {
-                unsigned acc = run[threadIdx.x];
+                unsigned acc = run[e];
                 #pragma unroll
                 for (int w = 0; w < kSortThreads / 64; ++w)
                 {
-                    wbase[w][threadIdx.x] = acc;
-                    acc += wcnt[w][threadIdx.x];
-                    wcnt[w][threadIdx.x] = 0;
+                    wbase[w][e] = acc;
+                    acc += wcnt[w][e];
+                    wcnt[w][e] = 0;
                 }
-                run[threadIdx.x] = acc;
+                run[e] = acc;
             }
             __syncthreads();
             if (valid)
@@ -629,7 +639,6 @@ void k_sort_scatter(const unsigned* __restrict__ keys_in,
                 keys_out[dst] = key;
                 vals_out[dst] = val;
             }
-            (void)lane;
         }
         __syncthreads();
     }
@@ -719,7 +728,7 @@ __global__ void k_gather_visits(const unsigned* __restrict__ vals,
         const unsigned* __restrict__ run_flag, const unsigned* __restrict__ run_excl,
         const unsigned* __restrict__ run_start, unsigned min_run, int max_w,
         const unsigned long long* __restrict__ d_n, int4* __restrict__ vmeta,
-        typename V4<T>::t* __restrict__ vamp)
+        typename V2<T>::t* __restrict__ vamp)
 {
     const unsigned long long n = *d_n;
     for (unsigned long long p = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
@@ -732,7 +741,8 @@ __global__ void k_gather_visits(const unsigned* __restrict__ vals,
             m.z |= kVisitStaged;
         if (run_flag[p]) m.z |= kVisitRunStart;
         vmeta[p] = m;
-        vamp[p] = amp[i];
+        const typename V4<T>::t am = amp[i];
+        vamp[p] = {am.x, am.z};            // the weighted amplitude (re, im)
     }
 }
 
@@ -790,7 +800,7 @@ struct GridArgs
     unsigned meta_bytes;           // meta (and amp) records, < 2^32
     unsigned visit_bytes;          // visit_vis
     const int4* vmeta;             // meta / amp of each sorted visit (strip kernel)
-    const typename V4<T>::t* vamp;
+    const typename V2<T>::t* vamp;        // weighted amplitude per sorted visit
     unsigned vrec_bytes;           // unused
     unsigned run_bytes;            // run_start / run_z / run_tb allocations
     unsigned item_group;           // strip kernel: consecutive items per XCD (>= 1)

@@ -395,30 +395,43 @@ int scan_u32(mi355_ctx* C, const unsigned* in, unsigned* out, DevCount cnt,
 enum Scalar { SC_VISITS, SC_OVERFLOW, SC_ITEMS, SC_SLOTS, SC_SCRATCH0, SC_RUNS = 6,
               SC_W_COUNT = 8, SC_W_BOX = 10, SC_COUNT = 12 };
 
-// Stable LSD radix sort, 8 bits a pass, of the pairs (C->keys[0], C->vals[0])
-// on the low key_bits bits of the keys: *d_n pairs, n_bound an upper bound of
-// *d_n known on the host.  *cur receives the index of the ping-pong buffers
-// holding the result.
+// Stable LSD radix sort of the pairs (C->keys[0], C->vals[0]) on the low
+// key_bits bits of the keys: *d_n pairs, n_bound an upper bound of *d_n
+// known on the host.  Digits are 8 bits, or 9 where that saves a pass (27
+// bits in 3 passes instead of 4).  *cur receives the index of the
+// ping-pong buffers holding the result.
 int radix_sort(mi355_ctx* C, unsigned long long n_bound, const unsigned long long* d_n,
         int key_bits, int* cur_out)
 {
     unsigned long long* sc = C->scalars.as<unsigned long long>();
+    const int bits = (key_bits + 8) / 9 < (key_bits + 7) / 8 ? 9 : 8;
+    const unsigned radix = 1u << bits;
     const unsigned long long nchunk = (n_bound + kSortChunk - 1) / kSortChunk;
-    TRY(ensure(C->hist, 256 * nchunk * sizeof(unsigned)));
-    TRY(ensure(C->hist_off, (256 * nchunk + 1) * sizeof(unsigned)));
+    TRY(ensure(C->hist, radix * nchunk * sizeof(unsigned)));
+    TRY(ensure(C->hist_off, (radix * nchunk + 1) * sizeof(unsigned)));
     int cur = 0;
-    for (int shift = 0; shift < key_bits; shift += 8)
+    for (int shift = 0; shift < key_bits; shift += bits)
     {
         const unsigned blocks = grid_blocks(nchunk, 1, 16384);
-        hipLaunchKernelGGL(k_sort_hist, dim3(blocks), dim3(kSortThreads), 0, C->stream,
-                C->keys[cur].as<unsigned>(), d_n, shift, C->hist.as<unsigned>());
-        DevCount hc{d_n, (unsigned long long)kSortChunk, 256, 0};
-        TRY(scan_u32(C, C->hist.as<unsigned>(), C->hist_off.as<unsigned>(), hc, 256 * nchunk,
+        if (bits == 9)
+            hipLaunchKernelGGL(k_sort_hist<9>, dim3(blocks), dim3(kSortThreads), 0, C->stream,
+                    C->keys[cur].as<unsigned>(), d_n, shift, C->hist.as<unsigned>());
+        else
+            hipLaunchKernelGGL(k_sort_hist<8>, dim3(blocks), dim3(kSortThreads), 0, C->stream,
+                    C->keys[cur].as<unsigned>(), d_n, shift, C->hist.as<unsigned>());
+        DevCount hc{d_n, (unsigned long long)kSortChunk, radix, 0};
+        TRY(scan_u32(C, C->hist.as<unsigned>(), C->hist_off.as<unsigned>(), hc, radix * nchunk,
                 sc + SC_SCRATCH0, reinterpret_cast<unsigned*>(sc + SC_SCRATCH0 + 1), 0));
-        hipLaunchKernelGGL(k_sort_scatter, dim3(blocks), dim3(kSortThreads), 0, C->stream,
-                C->keys[cur].as<unsigned>(), C->vals[cur].as<unsigned>(), d_n, shift,
-                C->hist_off.as<unsigned>(), C->keys[cur ^ 1].as<unsigned>(),
-                C->vals[cur ^ 1].as<unsigned>());
+        if (bits == 9)
+            hipLaunchKernelGGL(k_sort_scatter<9>, dim3(blocks), dim3(kSortThreads), 0, C->stream,
+                    C->keys[cur].as<unsigned>(), C->vals[cur].as<unsigned>(), d_n, shift,
+                    C->hist_off.as<unsigned>(), C->keys[cur ^ 1].as<unsigned>(),
+                    C->vals[cur ^ 1].as<unsigned>());
+        else
+            hipLaunchKernelGGL(k_sort_scatter<8>, dim3(blocks), dim3(kSortThreads), 0, C->stream,
+                    C->keys[cur].as<unsigned>(), C->vals[cur].as<unsigned>(), d_n, shift,
+                    C->hist_off.as<unsigned>(), C->keys[cur ^ 1].as<unsigned>(),
+                    C->vals[cur ^ 1].as<unsigned>());
         HIP_TRY(hipGetLastError());
         cur ^= 1;
     }
@@ -710,11 +723,11 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     if (strips)
     {
         TRY(ensure(C->vmeta, T_vis * sizeof(int4)));
-        TRY(ensure(C->vamp, T_vis * sizeof(T4)));
+        TRY(ensure(C->vamp, T_vis * sizeof(T2)));
         hipLaunchKernelGGL(k_gather_visits<T>, dim3(grid_blocks(T_vis, 256, 65535)), dim3(256), 0,
                 C->stream, svals, C->meta.as<int4>(), C->amp.as<T4>(), run_flag, run_excl,
                 C->run_start.as<unsigned>(), C->min_stage_run, kStripMaxW, sc + SC_VISITS,
-                C->vmeta.as<int4>(), C->vamp.as<T4>());
+                C->vmeta.as<int4>(), C->vamp.as<T2>());
         HIP_TRY(hipGetLastError());
     }
 
@@ -798,7 +811,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
         return fail(MI355_ERR_TOO_LARGE, "more than 2^30 tile visits in one call");
     ga.run_bytes = (unsigned)std::min<unsigned long long>((T_vis + 1) * 4ull, 0xFFFFFFFFull);
     ga.vmeta = C->vmeta.as<int4>();
-    ga.vamp = C->vamp.as<T4>();
+    ga.vamp = C->vamp.as<T2>();
     ga.vrec_bytes = 0;   // (the strip kernel sizes its record windows per item)
     ga.item_group = C->item_group;
     const unsigned padded = (unsigned)((num_items + 7) & ~7ull);
