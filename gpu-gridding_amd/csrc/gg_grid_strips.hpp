@@ -63,16 +63,21 @@ constexpr int kStripPad = 7;      // zero rows above and below a staged table
 // never multiplied), the 16 taps right before row 0 are zero (columns -15..-1
 // of row 0 are multiplied).
 constexpr int kStripGuard = 8 * kStripS + 16;
-constexpr int kStripLds = 20480;  // float2 taps: 160 KiB
+constexpr int kStripLds = 20448;  // float2 taps: 160 KiB less the prefetch area
 constexpr int kStripMaxW = 111;   // widest table staged
 #ifndef GG_DIRECT_GROUP
 #define GG_DIRECT_GROUP (GG_STRIP_WAVES >= 16 ? 1 : 8)
+#endif
+#ifndef GG_STRIP_PREFETCH
+#define GG_STRIP_PREFETCH 1
 #endif
 constexpr int kDirectGroup = GG_DIRECT_GROUP;   // chunks of global taps loaded together
 static_assert(kStripGuard - 8 * kStripS - 15 >= 0, "rows -8.. must stay inside the LDS array");
 static_assert(kStripGuard + (kStripMaxW + 21) * kStripS + kStripMaxW + 15 < kStripLds,
               "reads of the widest staged table must stay inside the LDS array");
 static_assert(kStripS - 15 >= kStripMaxW, "columns -15..-1 must land on zero columns");
+static_assert(kStripGuard + (kStripMaxW + 2 * kStripPad - 1) * kStripS + 128 <= kStripLds,
+              "staging of the widest table must stay inside the LDS array");
 static_assert(kTablePadRows >= 7, "direct reads of a touched chunk go 7 rows past a table");
 // Direct read-ahead goes up to 15 rows past a table: the table buffer has
 // kTableGuardRows rows of the widest table before the first and after the
@@ -93,41 +98,55 @@ __device__ __forceinline__ X rb_x4(__amdgpu_buffer_rsrc_t rs, unsigned off)
 
 // Copy a table into LDS at stride kStripS: staged row t + kStripPad,
 // column c holds table row t, column c, for t = -7 .. W+6 (the pad rows of
-// the padded block are zero); columns W .. 127 are zero (out-of-range
-// loads) and columns 128 .. kStripS-1 stay zero from the kernel's start (no
-// staged table reaches them).  Visits of all four orientations read this one
-// copy: mirrored columns and rows are in their LDS addresses.
+// the padded block are zero); columns W .. 127 are zero and columns 128 ..
+// kStripS-1 stay zero from the kernel's start (no staged table reaches
+// them).  Visits of all four orientations read this one copy: mirrored
+// columns and rows are in their LDS addresses.
 //
 // The copy goes straight from memory to LDS (buffer loads with the LDS
 // flag): no registers hold the data, so every row's loads are issued at
 // once and a table costs one round trip to memory.  Wave w copies rows w,
-// w + kStripWaves, ...; per row, four dword loads of the wave cover columns
-// 0 .. 127, lane l moving component l & 1 of column 32 k + l / 2.  The
-// caller's barrier after the copy waits for the loads (vmcnt) before any
-// wave reads LDS.
+// w + kStripWaves, ...; one 16-byte load per lane and row moves columns
+// 2l, 2l+1 to LDS (columns 0 .. 127 per row).  Lanes with 2l >= W load
+// nothing and write zeros; lane s (2s = W-1) also brings the tap after the
+// row's last (the next row's first), which one zero store per row then
+// clears.  The caller's barrier after the copy orders it before any read.
 typedef __attribute__((address_space(3))) void lds_void_t;
 static_assert(kStripMaxW < 128, "staged columns 128 .. kStripS-1 are always zero");
+static_assert((kStripGuard * 8) % 16 == 0 && (kStripS * 8) % 16 == 0,
+              "staged rows must start on 16 bytes");
 
 __device__ __forceinline__ void stage_strip_table(float2* tab, __amdgpu_buffer_rsrc_t rs,
         int tb, int W, int wave, int lane)
 {
     const int rows = W + 2 * kStripPad;
-    unsigned vo[4];
-    #pragma unroll
-    for (int k = 0; k < 4; ++k)
-    {
-        const int c = 32 * k + (lane >> 1);
-        vo[k] = c < W ? (unsigned)(c * 8 + (lane & 1) * 4) : 0x80000000u;
-    }
+    const unsigned vo = 2 * lane < W ? (unsigned)lane * 16u : 0x80000000u;
     for (int r = wave; r < rows; r += kStripWaves)
     {
         const unsigned so = (unsigned)((tb + (r - kStripPad) * W) * 8);
-        float2* row = tab + kStripGuard + r * kStripS;
-        #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void_t*)(row + 32 * k), 4, vo[k],
-                    so, 0, 0);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void_t*)(tab + kStripGuard + r * kStripS),
+                16, vo, so, 0, 0);
     }
+    // column W of this wave's rows (lane k: row wave + k kStripWaves); the
+    // store waits for the loads above (they write the same LDS array)
+    const int r = wave + lane * kStripWaves;
+    if (r < rows) tab[kStripGuard + r * kStripS + W] = make_float2(0.f, 0.f);
+}
+
+// Pull a table's block (pad rows included) towards the compute units ahead
+// of its staging: one dword per 64 bytes, so that a wave's load touches 4 KiB
+// and a 70-wide table takes 10 loads shared by the waves.  The loads go
+// through the LDS path into a 256-byte array of their own, so they hold no
+// registers, and reads of the staged table need not wait for them (it is a
+// different LDS object); the next staging's waits cover them.
+__device__ __forceinline__ void prefetch_strip_table(float* sink, __amdgpu_buffer_rsrc_t rs,
+        int tb, int W, int wave, int lane)
+{
+    const unsigned b0 = (unsigned)((tb - kTablePadRows * W) * 8);
+    const unsigned len = (unsigned)((W + 2 * kTablePadRows) * W * 8);
+    for (unsigned o = (unsigned)wave * 4096u; o < len; o += kStripWaves * 4096u)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void_t*)sink, 4,
+                o + (unsigned)lane * 64u, b0, 0, 0);
 }
 
 __device__ __forceinline__ void mac_strip(float& re, float& im, const float2 c, const float4 a)
@@ -200,6 +219,7 @@ __global__ __launch_bounds__(kStripBlock) __attribute__((amdgpu_waves_per_eu(kSt
 void k_grid_strips(const GridArgs<float> A)
 {
     __shared__ float2 tab[kStripLds];
+    __shared__ float pf_sink[64];                 // prefetch loads land here
     const unsigned num_items = ld_const(A.d_num_items, 0);
     const int lane = lane_id();
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -391,6 +411,14 @@ void k_grid_strips(const GridArgs<float> A)
                     __syncthreads();
                     staged_base = rtb;
                 }
+#if GG_STRIP_PREFETCH
+                if (starts)           // the batch's next run: its table heads for L2
+                {
+                    const int l1 = __builtin_ctzll(starts);
+                    prefetch_strip_table(pf_sink, rs_tab, bcast(mv.w, l1),
+                                         2 * meta_support(bcast(mv.z, l1)) + 1, wave, lane);
+                }
+#endif
                 const unsigned long long seg = meets & upto & ~((1ull << l0) - 1ull);
                 // visits whose rows run forwards, then those running backwards
                 unsigned long long todo = seg & ~vneg;

@@ -146,7 +146,7 @@ struct mi355_ctx
     bool timing = false;
     bool cache_kernels = false;
     int grid_kernel = 0;            // 0 auto (strips for plain FP32), 1 direct 64x64, 2 staged 64x64
-    unsigned min_stage_run = 6;
+    unsigned min_stage_run = 4;
     unsigned item_group = 8;        // strip kernel: consecutive work items per XCD
     int num_cu = 256;
 

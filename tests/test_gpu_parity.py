@@ -196,6 +196,7 @@ def test_kernel_variants_against_fp64_oracle(gpu, kernel, min_run):
     c = small_config(2, 30_000, 1024, 80, 72, seed=7)
     t, tn, ts = oracle.grid_fsel_d(c)
     ctx = ops.context()
+    msr0 = ctx.get_option(_lib.OPT_MIN_STAGE_RUN)
     ctx.set_option(_lib.OPT_GRID_KERNEL, kernel)
     ctx.set_option(_lib.OPT_MIN_STAGE_RUN, min_run)
     try:
@@ -203,7 +204,7 @@ def test_kernel_variants_against_fp64_oracle(gpu, kernel, min_run):
         g2, _, _ = run_grid(torch, ops, c, ctx=ctx, visits_per_item=53)
     finally:
         ctx.set_option(_lib.OPT_GRID_KERNEL, 0)
-        ctx.set_option(_lib.OPT_MIN_STAGE_RUN, 6)
+        ctx.set_option(_lib.OPT_MIN_STAGE_RUN, msr0)
     assert s == ts and abs(n - tn) <= 1e-12 * abs(tn)
     emax, el2 = errors(g, t)
     assert emax <= 1e-5 and el2 <= 1e-5, (kernel, min_run, emax, el2)
