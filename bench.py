@@ -276,7 +276,7 @@ def main():
                     help="gridding kernel: 16-column strips on 128x128 regions, or 64x64 "
                          "tiles with tables staged in LDS / read directly")
     ap.add_argument("--visits-per-item", type=int, default=2048)
-    ap.add_argument("--min-stage-run", type=int, default=4)
+    ap.add_argument("--min-stage-run", type=int, default=3)
     ap.add_argument("--item-group", type=int, default=8,
                     help="strip kernel: consecutive work items dealt to one XCD")
     ap.add_argument("--dropin", action="store_true",

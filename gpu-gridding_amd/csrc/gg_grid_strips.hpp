@@ -40,11 +40,13 @@ namespace gg {
 constexpr int kStripW = 16;                        // columns per wave
 constexpr int kStripRows = 128;                    // rows per wave
 // Waves per workgroup (one workgroup per CU: the staged table takes the LDS)
-// and chunks of global taps loaded together.  Default: 12 waves (three per
-// SIMD, up to 168 VGPRs) with 8 chunks; at 16 waves (128 VGPRs) the 64
-// accumulators leave room for 1 chunk.
+// and chunks of global taps loaded together.  Default: 16 waves (four per
+// SIMD, up to 128 VGPRs, 64 of them accumulators) with 2 chunks; 12 waves
+// (three per SIMD, up to 168 VGPRs) take 8 chunks.  Four waves per SIMD hide
+// more of the staged pass's LDS latency (its time alone: 53.7 ms against
+// 60.5 ms at 12 waves) for a slightly slower direct pass.
 #ifndef GG_STRIP_WAVES
-#define GG_STRIP_WAVES 12
+#define GG_STRIP_WAVES 16
 #endif
 constexpr int kStripWaves = GG_STRIP_WAVES;
 constexpr int kStripBlock = kStripWaves * 64;
@@ -63,10 +65,10 @@ constexpr int kStripPad = 7;      // zero rows above and below a staged table
 // never multiplied), the 16 taps right before row 0 are zero (columns -15..-1
 // of row 0 are multiplied).
 constexpr int kStripGuard = 8 * kStripS + 16;
-constexpr int kStripLds = 20448;  // float2 taps: 160 KiB less the prefetch area
+constexpr int kStripLds = 20446;  // float2 taps: 160 KiB less the prefetch area and a counter
 constexpr int kStripMaxW = 111;   // widest table staged
 #ifndef GG_DIRECT_GROUP
-#define GG_DIRECT_GROUP (GG_STRIP_WAVES >= 16 ? 1 : 8)
+#define GG_DIRECT_GROUP (GG_STRIP_WAVES >= 16 ? 2 : 8)
 #endif
 #ifndef GG_STRIP_PREFETCH
 #define GG_STRIP_PREFETCH 1
@@ -237,16 +239,28 @@ void k_grid_strips(const GridArgs<float> A)
     __syncthreads();
     int staged_base = -1;                          // table in LDS (uniform)
 
-    // Position p runs on XCD p % 8 (the launch is a multiple of 8
-    // workgroups, dealt out round-robin).  Within each block of 8g positions
-    // XCD x takes the g consecutive items x*g .. x*g+g-1 (g = item_group), so
-    // neighbouring regions -- which share the tables of the visibilities
-    // whose footprints straddle them -- run on one L2, while all XCDs still
-    // advance through the item list together.
+    // A persistent grid (a multiple of 8 workgroups, dealt out to the XCDs
+    // round-robin): workgroup b takes the next free position of its XCD,
+    // x = b mod 8, from that XCD's counter in item_ctr -- positions x, x+8,
+    // x+16, ... in turn -- so that uneven items balance over the CUs and a
+    // staged table stays in LDS for the next item.  Within each block of 8g
+    // positions XCD x takes the g consecutive items x*g .. x*g+g-1 (g =
+    // item_group), so neighbouring regions -- which share the tables of the
+    // visibilities whose footprints straddle them -- run on one L2, while all
+    // XCDs still advance through the item list together.
     const unsigned g = A.item_group, blk = 8u * g;
     const unsigned npos = (num_items + blk - 1) / blk * blk;
-    for (unsigned p = blockIdx.x; p < npos; p += gridDim.x)
+    __shared__ unsigned s_next;
+    for (;;)
     {
+        {
+            const unsigned x = blockIdx.x & 7u;
+            if (threadIdx.x == 0) s_next = atomicAdd(A.item_ctr + x, 1u) * 8u + x;
+        }
+        __syncthreads();
+        const unsigned p = __builtin_amdgcn_readfirstlane(s_next);
+        __syncthreads();                   // (read by every wave before the next write)
+        if (p >= npos) break;
         const unsigned item = p / blk * blk + (p & 7u) * g + (p >> 3) % g;
         if (item >= num_items) continue;
         const unsigned t = ld_const(A.item_tile, item);

@@ -804,6 +804,8 @@ struct GridArgs
     unsigned vrec_bytes;           // unused
     unsigned run_bytes;            // run_start / run_z / run_tb allocations
     unsigned item_group;           // strip kernel: consecutive items per XCD (>= 1)
+    unsigned* item_ctr;            // strip kernel: 8 zeroed counters (positions
+                                   // taken per XCD)
 };
 
 // TwoSum (Knuth): s + e == a + b exactly.

@@ -146,7 +146,7 @@ struct mi355_ctx
     bool timing = false;
     bool cache_kernels = false;
     int grid_kernel = 0;            // 0 auto (strips for plain FP32), 1 direct 64x64, 2 staged 64x64
-    unsigned min_stage_run = 4;
+    unsigned min_stage_run = 3;
     unsigned item_group = 8;        // strip kernel: consecutive work items per XCD
     int num_cu = 256;
 
@@ -393,7 +393,7 @@ int scan_u32(mi355_ctx* C, const unsigned* in, unsigned* out, DevCount cnt,
 // their own slots (count and bounding box), so that they leave the counts of
 // a gridding call, read later by mi355_ctx_get_stats, in place.
 enum Scalar { SC_VISITS, SC_OVERFLOW, SC_ITEMS, SC_SLOTS, SC_SCRATCH0, SC_RUNS = 6,
-              SC_W_COUNT = 8, SC_W_BOX = 10, SC_COUNT = 12 };
+              SC_W_COUNT = 8, SC_W_BOX = 10, SC_ITEM_CTR = 12, SC_COUNT = 16 };
 
 // Stable LSD radix sort of the pairs (C->keys[0], C->vals[0]) on the low
 // key_bits bits of the keys: *d_n pairs, n_bound an upper bound of *d_n
@@ -814,10 +814,16 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     ga.vamp = C->vamp.as<T2>();
     ga.vrec_bytes = 0;   // (the strip kernel sizes its record windows per item)
     ga.item_group = C->item_group;
+    // the strip kernel's persistent grid takes positions from per-XCD
+    // counters (zeroed with the other scalars at the start of the pass)
+    ga.item_ctr = reinterpret_cast<unsigned*>(sc + SC_ITEM_CTR);
     const unsigned padded = (unsigned)((num_items + 7) & ~7ull);
-    // One workgroup per work item, in item order (consecutive items of a
-    // heavy tile land on different XCDs); host-sync-free, a persistent grid
-    // of two workgroups per CU walks the items the device counted.
+    // The strip kernel: a persistent grid of one workgroup per CU that takes
+    // item positions from per-XCD counters, so that uneven items balance and
+    // a staged table stays in LDS for the next item.  The 64x64 kernels: one
+    // workgroup per work item, in item order (consecutive items of a heavy
+    // tile land on different XCDs); host-sync-free, a persistent grid of two
+    // workgroups per CU walks the items the device counted.
     const unsigned wg_items = C->sync_free
             ? (unsigned)std::min<unsigned long long>(num_items, 2ull * C->num_cu)
             : (unsigned)num_items;
@@ -827,8 +833,8 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
         {
             // (a multiple of 8 workgroups: the kernel maps positions to XCDs)
             const unsigned long long blk = 8ull * C->item_group;
-            const unsigned wg8 = C->sync_free ? std::max(8u, wg_items & ~7u)
-                    : (unsigned)((num_items + blk - 1) / blk * blk);
+            const unsigned wg8 = (unsigned)std::min<unsigned long long>(
+                    std::max(8u, (unsigned)C->num_cu & ~7u), (num_items + blk - 1) / blk * blk);
             hipLaunchKernelGGL(k_grid_strips, dim3(wg8), dim3(kStripBlock), 0, C->stream, ga);
             HIP_TRY(hipGetLastError());
         }
@@ -1820,7 +1826,7 @@ MI355_EXPORT int mi355_autotune_f_batched_dev(mi355_ctx* C, size_t P, const int*
     std::vector<Cand> cands;
     for (int kern : {0, 2})
         for (unsigned vpi : {1024u, 2048u, 4096u, 8192u})
-            for (unsigned msr : {2u, 4u, 6u, 8u})
+            for (unsigned msr : {2u, 3u, 4u, 6u})
                 for (unsigned grp : {1u, 8u})
                     if (kern == 0 || (msr == 4u && grp == ig0)) cands.push_back({kern, vpi, msr, grp});
     double best = 1e300;

@@ -102,7 +102,7 @@ MI355_EXPORT int mi355_ctx_set_stream(mi355_ctx* ctx, void* hip_stream);
                                         192x128 regions for plain FP32, else 2;
                                         1: 64x64 tiles, tables read directly;
                                         2: 64x64 tiles, tables staged in LDS */
-#define MI355_OPT_MIN_STAGE_RUN   6  /* shortest same-table run staged in LDS (default 4) */
+#define MI355_OPT_MIN_STAGE_RUN   6  /* shortest same-table run staged in LDS (default 3) */
 #define MI355_OPT_ITEM_GROUP      7  /* strip kernel: work items are dealt to the
                                         8 XCDs in groups of this many consecutive
                                         items (default 8; 1: round-robin) */
