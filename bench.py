@@ -275,9 +275,9 @@ def main():
     ap.add_argument("--kernel", choices=("strips", "staged", "direct"), default="strips",
                     help="gridding kernel: 16-column strips on 128x128 regions, or 64x64 "
                          "tiles with tables staged in LDS / read directly")
-    ap.add_argument("--visits-per-item", type=int, default=2048)
+    ap.add_argument("--visits-per-item", type=int, default=4096)
     ap.add_argument("--min-stage-run", type=int, default=3)
-    ap.add_argument("--item-group", type=int, default=8,
+    ap.add_argument("--item-group", type=int, default=1,
                     help="strip kernel: consecutive work items dealt to one XCD")
     ap.add_argument("--dropin", action="store_true",
                     help="also time the host-pointer drop-in (PCIe included)")

@@ -141,13 +141,13 @@ struct mi355_ctx
 {
     int device = 0;
     hipStream_t stream = nullptr;
-    unsigned visits_per_item = 2048;
+    unsigned visits_per_item = 4096;
     bool compensated = false;
     bool timing = false;
     bool cache_kernels = false;
     int grid_kernel = 0;            // 0 auto (strips for plain FP32), 1 direct 64x64, 2 staged 64x64
     unsigned min_stage_run = 3;
-    unsigned item_group = 8;        // strip kernel: consecutive work items per XCD
+    unsigned item_group = 1;        // strip kernel: consecutive work items per XCD
     int num_cu = 256;
 
     // Host-sync-free mode (buffers sized from upper bounds, counts read on

@@ -93,7 +93,7 @@ MI355_EXPORT int mi355_default_ctx(mi355_ctx** ctx);
 MI355_EXPORT int mi355_ctx_set_stream(mi355_ctx* ctx, void* hip_stream);
 
 /* Tuning and behaviour options (mi355_ctx_set_option). */
-#define MI355_OPT_VISITS_PER_ITEM 1  /* tile visits per work item (default 2048) */
+#define MI355_OPT_VISITS_PER_ITEM 1  /* tile visits per work item (default 4096) */
 #define MI355_OPT_COMPENSATED     2  /* 1: TwoSum-compensated LDS accumulation */
 #define MI355_OPT_PHASE_TIMING    3  /* 1: record hipEvents around each phase */
 #define MI355_OPT_CACHE_KERNELS   4  /* 1: reuse prepared kernel tables while the
@@ -105,7 +105,7 @@ MI355_EXPORT int mi355_ctx_set_stream(mi355_ctx* ctx, void* hip_stream);
 #define MI355_OPT_MIN_STAGE_RUN   6  /* shortest same-table run staged in LDS (default 3) */
 #define MI355_OPT_ITEM_GROUP      7  /* strip kernel: work items are dealt to the
                                         8 XCDs in groups of this many consecutive
-                                        items (default 8; 1: round-robin) */
+                                        items (default 1: round-robin) */
 MI355_EXPORT int mi355_ctx_set_option(mi355_ctx* ctx, int key, long long value);
 MI355_EXPORT int mi355_ctx_get_option(const mi355_ctx* ctx, int key, long long* value);
 

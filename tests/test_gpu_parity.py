@@ -47,6 +47,7 @@ def run_grid(torch, ops, c, dtype=None, mode="plain", ctx=None, visits_per_item=
     norm = torch.zeros(1, dtype=torch.float64, device="cuda")
     skipped = torch.zeros(1, dtype=torch.int64, device="cuda")
     ctx = ctx or ops.context()
+    vpi0 = ctx.get_option(_lib.OPT_VISITS_PER_ITEM)
     ctx.set_option(_lib.OPT_COMPENSATED, 1 if mode == "comp" else 0)
     if visits_per_item:
         ctx.set_option(_lib.OPT_VISITS_PER_ITEM, visits_per_item)
@@ -59,7 +60,7 @@ def run_grid(torch, ops, c, dtype=None, mode="plain", ctx=None, visits_per_item=
                      c["w_scale"], G, grid, norm, skipped, ctx=ctx)
     finally:
         ctx.set_option(_lib.OPT_COMPENSATED, 0)
-        ctx.set_option(_lib.OPT_VISITS_PER_ITEM, 2048)
+        ctx.set_option(_lib.OPT_VISITS_PER_ITEM, vpi0)
     torch.cuda.synchronize()
     return grid.cpu().numpy(), float(norm.item()), int(skipped.item())
 
