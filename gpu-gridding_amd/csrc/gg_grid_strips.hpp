@@ -225,13 +225,13 @@ void k_grid_strips(const GridArgs<float> A)
     const unsigned num_items = ld_const(A.d_num_items, 0);
     const int lane = lane_id();
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int lr = lane >> 4, lc = lane & 15;
-    // LDS row of this lane's table row when the visit's footprint starts at
-    // strip row 0: staged row kStripPad + lr, or kStripPad - lr (plus the
-    // visit's offset) when its rows run backwards; the column (lc, or
-    // backwards when mirrored in u) is added per visit.
-    const int lane_lds = kStripGuard + (kStripPad + lr) * kStripS;
-    const int lane_lds_m = kStripGuard + (kStripPad - lr) * kStripS;
+    // Each pass and the write-out take the lane's row and column in its slots
+    // (lr, lc) from their own copy of the lane id, hidden from the compiler,
+    // so that nothing derived from it is kept live across the item loop.
+#define GG_LANE_RC                                                              \
+        int ln_ = lane;                                                         \
+        __asm__ volatile("" : "+v"(ln_));                                       \
+        const int lr = ln_ >> 4, lc = ln_ & 15;
     const __amdgpu_buffer_rsrc_t rs_tab = table_rsrc(A.tables, A.table_bytes);
     // all of LDS zero: the 16 taps before staged row 0 and columns 128 ..
     // kStripS-1 of every row are read as zeros and never written again
@@ -328,6 +328,8 @@ void k_grid_strips(const GridArgs<float> A)
         const int q1 = min(top + 2 * s, kStripRows - 1) >> 3;                   \
         const int cm = (int)((2u << q1) - (1u << q0));
 
+        {
+        GG_LANE_RC
         // Pass 1: visits of runs read straight from the tables in global
         // memory.  A barrier per batch keeps the waves on the same visits, so
         // the cache lines that neighbouring strips' row segments share are
@@ -390,7 +392,24 @@ void k_grid_strips(const GridArgs<float> A)
                 }
             }
         }
-
+        }
+        {
+        GG_LANE_RC
+        // LDS row of this lane's table row when the visit's footprint starts at
+        // strip row 0: staged row kStripPad + lr, or kStripPad - lr (plus the
+        // visit's offset) when its rows run backwards; the column (lc, or
+        // backwards when mirrored in u) is added per visit.
+        const int lane_lds = kStripGuard + (kStripPad + lr) * kStripS;
+        const int lane_lds_m = kStripGuard + (kStripPad - lr) * kStripS;
+        // The lane's part of a visit's LDS byte offsets (the visit's part,
+        // from its record, is added per visit): forwards / backwards in u,
+        // rows running forwards (f) or backwards (r); the strip's origin
+        // (xs, ys) enters with the sign of the column and row directions.
+        const int bpf = 8 * (lane_lds + lc + xs + ys * kStripS);
+        const int bnf = 8 * (lane_lds - lc - xs + ys * kStripS);
+        const int bpr = 8 * (lane_lds_m + lc + xs - (ys + 60) * kStripS);
+        const int bnr = 8 * (lane_lds_m - lc - xs - (ys + 60) * kStripS);
+        const __amdgpu_buffer_rsrc_t rs_vlds = table_rsrc(A.vlds + v0, (v1 - v0) * 4u);
         // Pass 2: visits of staged runs, 64 at a time.  The batch's staged
         // visits split into segments at run starts (the first segment may
         // continue a run of the previous batch); each segment's table is
@@ -402,6 +421,7 @@ void k_grid_strips(const GridArgs<float> A)
             const unsigned pl = b + (unsigned)lane;
             const int4 mv = rb_x4<int4>(rs_vmeta, (pl - v0) * 16u);
             const float2 av = load_tap(rs_vamp, (pl - v0) * 8u, 0u, (float2*)nullptr);
+            const int vl = (int)rb_u32(rs_vlds, (pl - v0) * 4u);
             const bool stg = pl < v1 && (mv.z & kVisitStaged);
             const unsigned long long stg_mask = __ballot(stg);
             if (!stg_mask) continue;
@@ -441,14 +461,16 @@ void k_grid_strips(const GridArgs<float> A)
                     const int l = __builtin_ctzll(todo);
                     todo &= todo - 1ull;
                     GG_VISIT_HEAD
-                    // table column of the lane: kc + lc, or W-1-kc-lc mirrored
-                    const int lcol = (z & kFlagNegU) ? (2 * s - kc) - lc : kc + lc;
-                    // (two independent bases: every block's reads then fit
-                    // the positive offset fields of ds_read2st64)
-                    int o0 = lane_lds + lcol - top * kStripS, o1 = o0 + 64 * kStripS;
+                    // tap of chunk 0: table column kc + lc (or W-1-kc-lc
+                    // mirrored), row -top + lr.  (Two independent bases:
+                    // every block's reads then fit the positive offset fields
+                    // of ds_read2st64.)
+                    const int u = bcast(vl, l);
+                    const int bs = (z & kFlagNegU) ? bnf : bpf;
+                    int o0 = bs + u, o1 = bs + (u + 8 * 64 * kStripS);
                     __asm__ volatile("" : "+v"(o0), "+v"(o1));
-                    const float2* t0 = tab + o0;
-                    const float2* t1 = tab + o1;
+                    const float2* t0 = (const float2*)((const char*)tab + o0);
+                    const float2* t1 = (const float2*)((const char*)tab + o1);
                     GG_STRIP_PAIRS(cm)
                 }
                 todo = seg & vneg;
@@ -457,19 +479,23 @@ void k_grid_strips(const GridArgs<float> A)
                     const int l = __builtin_ctzll(todo);
                     todo &= todo - 1ull;
                     GG_VISIT_HEAD
-                    // table row of strip row 0: W-1+top, slot i 4i rows up
-                    const int lcol = (z & kFlagNegU) ? (2 * s - kc) - lc : kc + lc;
-                    int o0 = lane_lds_m + lcol + (2 * s + top - 60) * kStripS,
-                        o1 = o0 - 64 * kStripS;
+                    // table row of strip row 0: W-1+top, slot i 4i rows up;
+                    // t0 / t1 at block 3 / 7 (60 rows up from strip row 0)
+                    const int u = bcast(vl, l);
+                    const int bs = (z & kFlagNegU) ? bnr : bpr;
+                    int o0 = bs + u, o1 = bs + (u - 8 * 64 * kStripS);
                     __asm__ volatile("" : "+v"(o0), "+v"(o1));
-                    const float2* t0 = tab + o0;
-                    const float2* t1 = tab + o1;
+                    const float2* t0 = (const float2*)((const char*)tab + o0);
+                    const float2* t1 = (const float2*)((const char*)tab + o1);
                     GG_STRIP_PAIRS_VMIRROR(cm)
                 }
             }
         }
+        }
 #undef GG_VISIT_HEAD
 
+        {
+        GG_LANE_RC
         // Strip out: lane (lr, lc) of slot i is cell (xs + lc, ys + 4i + lr).
         // The lane's base offset goes through an empty asm so that the 32
         // slot addresses are formed here rather than kept live (or spilled)
@@ -506,11 +532,13 @@ void k_grid_strips(const GridArgs<float> A)
             for (int i = 0; i < kStripSlots; ++i)
                 out[i * 4 * kRegionW] = make_float2(re[i], im[i]);
         }
+        }
         // (the staged table stays valid for the next item: only staging
         // writes LDS, after a barrier)
     }
 }
 
+#undef GG_LANE_RC
 #undef GG_STRIP_PAIRS
 #undef GG_STRIP_PAIRS_VMIRROR
 

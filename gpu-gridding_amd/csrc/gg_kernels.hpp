@@ -726,9 +726,9 @@ template <typename T>
 __global__ void k_gather_visits(const unsigned* __restrict__ vals,
         const int4* __restrict__ meta, const typename V4<T>::t* __restrict__ amp,
         const unsigned* __restrict__ run_flag, const unsigned* __restrict__ run_excl,
-        const unsigned* __restrict__ run_start, unsigned min_run, int max_w,
+        const unsigned* __restrict__ run_start, unsigned min_run, int max_w, int lds_stride,
         const unsigned long long* __restrict__ d_n, int4* __restrict__ vmeta,
-        typename V2<T>::t* __restrict__ vamp)
+        typename V2<T>::t* __restrict__ vamp, int* __restrict__ vlds)
 {
     const unsigned long long n = *d_n;
     for (unsigned long long p = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
@@ -743,6 +743,15 @@ __global__ void k_gather_visits(const unsigned* __restrict__ vals,
         vmeta[p] = m;
         const typename V4<T>::t am = amp[i];
         vamp[p] = {am.x, am.z};            // the weighted amplitude (re, im)
+        // The visit's part of its staged-tap LDS byte offsets (the strip
+        // kernel adds the lane's part): footprint column 0 is table column
+        // s - x + (strip column), or x + s - (strip column) mirrored in u;
+        // footprint row 0 is staged row -(y - s) + (strip row), or the rows
+        // run up from y + s when mirrored in v.
+        const int sp = meta_support(m.z);
+        const int cu = (m.z & kFlagNegU) ? m.x + sp : sp - m.x;
+        const int rv = (m.z & kFlagNegV) ? m.y + sp : sp - m.y;
+        vlds[p] = 8 * (cu + rv * lds_stride);
     }
 }
 
@@ -803,6 +812,7 @@ struct GridArgs
     const typename V2<T>::t* vamp;        // weighted amplitude per sorted visit
     unsigned vrec_bytes;           // unused
     unsigned run_bytes;            // run_start / run_z / run_tb allocations
+    const int* vlds;               // per sorted visit: its part of the staged LDS offsets
     unsigned item_group;           // strip kernel: consecutive items per XCD (>= 1)
     unsigned* item_ctr;            // strip kernel: 8 zeroed counters (positions
                                    // taken per XCD)

@@ -181,7 +181,7 @@ struct mi355_ctx
     DevBuf keys[2], vals[2], hist, hist_off;
     DevBuf tile_start, tile_end, n_items, n_slots, item_off, slot_off, item_tile;
     DevBuf run_start, run_z, run_tb;
-    DevBuf vmeta, vamp;             // sorted visit records (strip kernel)
+    DevBuf vmeta, vamp, vlds;       // sorted visit records (strip kernel)
     DevBuf scratch;
     DevBuf jobs_d, job_chunk_off_d, job_vis_off_d, job_tile_base_d, job_out_d, job_bbox_d;
     // drop-in staging
@@ -724,10 +724,11 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     {
         TRY(ensure(C->vmeta, T_vis * sizeof(int4)));
         TRY(ensure(C->vamp, T_vis * sizeof(T2)));
+        TRY(ensure(C->vlds, T_vis * sizeof(int)));
         hipLaunchKernelGGL(k_gather_visits<T>, dim3(grid_blocks(T_vis, 256, 65535)), dim3(256), 0,
                 C->stream, svals, C->meta.as<int4>(), C->amp.as<T4>(), run_flag, run_excl,
-                C->run_start.as<unsigned>(), C->min_stage_run, kStripMaxW, sc + SC_VISITS,
-                C->vmeta.as<int4>(), C->vamp.as<T2>());
+                C->run_start.as<unsigned>(), C->min_stage_run, kStripMaxW, kStripS, sc + SC_VISITS,
+                C->vmeta.as<int4>(), C->vamp.as<T2>(), C->vlds.as<int>());
         HIP_TRY(hipGetLastError());
     }
 
@@ -812,6 +813,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     ga.run_bytes = (unsigned)std::min<unsigned long long>((T_vis + 1) * 4ull, 0xFFFFFFFFull);
     ga.vmeta = C->vmeta.as<int4>();
     ga.vamp = C->vamp.as<T2>();
+    ga.vlds = C->vlds.as<int>();
     ga.vrec_bytes = 0;   // (the strip kernel sizes its record windows per item)
     ga.item_group = C->item_group;
     // the strip kernel's persistent grid takes positions from per-XCD
