@@ -1,36 +1,38 @@
 // gg_grid_strips.hpp -- FP32 region gridding, one wave per 16-column strip.
 //
-// A workgroup of twelve waves owns a 192 x 128 region of the grid; wave w
-// owns the strip of columns 16w .. 16w+15 of it, all 128 rows.  A lane owns
-// one column of four rows of each of 32 slots (lane = 16 * row-in-slot +
-// column), so the strip's 2048 cells are 32 complex accumulators per lane,
-// kept in registers.  A footprint of width F covers about F/16 + 1 strip
-// columns-blocks of 16, so few lanes idle at its left and right edges.
+// A workgroup of 16 waves owns a 256 x 128 region of the grid (12 waves and
+// 192 x 64 when compensated); wave w owns the strip of columns 16w .. 16w+15
+// of it, all its rows.  The strip's rows go in slots of four; a lane owns
+// one cell of each slot (lane = 16 * row-in-slot + column), so the strip's
+// cells are 32 (compensated: 16) complex accumulators per lane, kept in
+// registers.  A footprint of width F covers about F/16 + 1 strips, so few
+// lanes idle at its left and right edges.
 //
-// The region's visits are sorted by kernel table and orientation (mirror
-// flags), so its list is a sequence of runs that read one table in one
-// orientation.  A run of at least min_stage_run visits whose table is at
-// most kStripMaxW wide is staged in LDS once, already mirrored, at a fixed
-// row stride with zero rows and columns around it; every visit of the run
-// then reads a lane's tap of slot i at a constant offset from one per-visit
-// base address.  Other visits read taps straight from the padded tables in
-// global memory with raw buffer loads.
+// The region's visits are sorted by kernel table, so its list is a sequence
+// of runs that read one table (in any of the four mirror orientations).  A
+// run of at least min_stage_run visits whose table is at most kStripMaxW
+// wide is staged in LDS once, unmirrored, at a fixed row stride with zero
+// rows and columns around it; every visit of the run then reads a lane's tap
+// of slot i at a constant offset from one per-visit base address.  Other
+// visits read taps straight from the padded tables in global memory with
+// raw buffer loads.
 //
 // Rows go in chunks of eight (two slots).  A visit touches chunks q0..q1 of
 // the strip; a chunk is skipped with a wave-uniform branch.  From LDS the
-// four slots of a pair of chunks are read together before their multiplies;
-// from global memory the touched chunks are requested kDirectGroup at a time.  Rows of a touched chunk
-// outside the footprint read zero rows, columns outside it zero columns
-// (staged) or nothing (direct: the buffer offset is out of range).
+// four slots of a pair of chunks are read together before their
+// multiplies; from global memory the touched chunks are requested
+// kDirectGroup at a time.  Rows of a touched chunk outside the footprint
+// read zero rows, columns outside it zero columns (staged) or nothing
+// (direct: the buffer offset is out of range).
 //
 // Each wave walks all visits of a run segment 64 at a time: one vector load
 // per lane of the records, a ballot of the visits whose footprint meets its
 // strip, then those visits one by one with their records broadcast by
 // readlane and the per-visit addressing done in scalar registers.  Every
 // load in the loop nest is a raw buffer load (scalar base, 32-bit offset),
-// which keeps per-lane address registers to one per stream.  At the end of a work item the strip is added to the grid
-// (single-item regions) or stored in the item's partial slot (merged in item
-// order, FP64, afterwards).
+// which keeps per-lane address registers to one per stream.  At the end of
+// a work item the strip is added to the grid (single-item regions) or stored
+// in the item's partial slot (merged in item order, FP64, afterwards).
 #pragma once
 
 #include "gg_kernels.hpp"
@@ -38,24 +40,36 @@
 namespace gg {
 
 constexpr int kStripW = 16;                        // columns per wave
-constexpr int kStripRows = 128;                    // rows per wave
-// Waves per workgroup (one workgroup per CU: the staged table takes the LDS)
-// and chunks of global taps loaded together.  Default: 16 waves (four per
-// SIMD, up to 128 VGPRs, 64 of them accumulators) with 2 chunks; 12 waves
-// (three per SIMD, up to 168 VGPRs) take 8 chunks.  Four waves per SIMD hide
-// more of the staged pass's LDS latency (its time alone: 53.7 ms against
-// 60.5 ms at 12 waves) for a slightly slower direct pass.
+// Geometry of a strip kernel: WAVES waves per workgroup (one workgroup per
+// CU: the staged table takes the LDS), each owning 16 columns x ROWS rows,
+// and COMP for compensated accumulation.
+//  - Plain FP32: 16 waves (four per SIMD, up to 128 VGPRs, 64 of them
+//    accumulators) on 256 x 128 regions, 2 chunks of global taps loaded
+//    together.  Four waves per SIMD hide more of the staged pass's LDS
+//    latency than three at 12 waves (8 chunks, up to 168 VGPRs): the staged
+//    pass alone took 53.7 ms against 60.5 ms, for a slightly slower direct
+//    pass.
+//  - Compensated FP32: 12 waves on 192 x 64 regions (16 slots per lane:
+//    32 running sums, and 64 TwoSum sums and error terms), 4 chunks.
 #ifndef GG_STRIP_WAVES
 #define GG_STRIP_WAVES 16
 #endif
-constexpr int kStripWaves = GG_STRIP_WAVES;
-constexpr int kStripBlock = kStripWaves * 64;
-constexpr int kRegionW = kStripWaves * kStripW;    // 256 (or 192) columns
-constexpr int kStripWavesPerSimd = kStripWaves / 4;
-constexpr int kRegionH = kStripRows;               // 128 rows
-static_assert(kRegionW * kRegionH % kMergeCells == 0, "merge blocks tile a region");
-constexpr int kStripSlots = kStripRows / 4;        // 32 slots of 16 columns x 4 rows
-constexpr int kStripChunks = kStripSlots / 2;      // 16 chunks of 8 rows
+template <int WAVES, int ROWS, bool COMP>
+struct StripCfg
+{
+    static constexpr int kWaves = WAVES;
+    static constexpr int kBlock = WAVES * 64;
+    static constexpr int kRegionW = WAVES * kStripW;
+    static constexpr int kRegionH = ROWS;
+    static constexpr int kSlots = ROWS / 4;          // slots of 16 columns x 4 rows
+    static constexpr int kChunks = ROWS / 8;         // chunks of 8 rows
+    static constexpr int kDirectGroup = COMP ? 4 : (WAVES >= 16 ? 2 : 8);
+    static constexpr bool kComp = COMP;
+    static_assert(ROWS % 16 == 0 && ROWS <= 128, "blocks of 16 rows, two LDS bases of 64 rows");
+    static_assert(kRegionW * kRegionH % kMergeCells == 0, "merge blocks tile a region");
+};
+using StripPlain = StripCfg<GG_STRIP_WAVES, 128, false>;
+using StripComp = StripCfg<12, 64, true>;
 constexpr int kStripS = 144;      // LDS row stride in taps; 144 = 16 (mod 32): the two
                                   // 128-byte row segments of a half-wave use disjoint banks
 constexpr int kStripPad = 7;      // zero rows above and below a staged table
@@ -67,13 +81,9 @@ constexpr int kStripPad = 7;      // zero rows above and below a staged table
 constexpr int kStripGuard = 8 * kStripS + 16;
 constexpr int kStripLds = 20446;  // float2 taps: 160 KiB less the prefetch area and a counter
 constexpr int kStripMaxW = 111;   // widest table staged
-#ifndef GG_DIRECT_GROUP
-#define GG_DIRECT_GROUP (GG_STRIP_WAVES >= 16 ? 2 : 8)
-#endif
 #ifndef GG_STRIP_PREFETCH
 #define GG_STRIP_PREFETCH 1
 #endif
-constexpr int kDirectGroup = GG_DIRECT_GROUP;   // chunks of global taps loaded together
 static_assert(kStripGuard - 8 * kStripS - 15 >= 0, "rows -8.. must stay inside the LDS array");
 static_assert(kStripGuard + (kStripMaxW + 21) * kStripS + kStripMaxW + 15 < kStripLds,
               "reads of the widest staged table must stay inside the LDS array");
@@ -118,20 +128,21 @@ static_assert(kStripMaxW < 128, "staged columns 128 .. kStripS-1 are always zero
 static_assert((kStripGuard * 8) % 16 == 0 && (kStripS * 8) % 16 == 0,
               "staged rows must start on 16 bytes");
 
+template <int WAVES>
 __device__ __forceinline__ void stage_strip_table(float2* tab, __amdgpu_buffer_rsrc_t rs,
         int tb, int W, int wave, int lane)
 {
     const int rows = W + 2 * kStripPad;
     const unsigned vo = 2 * lane < W ? (unsigned)lane * 16u : 0x80000000u;
-    for (int r = wave; r < rows; r += kStripWaves)
+    for (int r = wave; r < rows; r += WAVES)
     {
         const unsigned so = (unsigned)((tb + (r - kStripPad) * W) * 8);
         __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void_t*)(tab + kStripGuard + r * kStripS),
                 16, vo, so, 0, 0);
     }
-    // column W of this wave's rows (lane k: row wave + k kStripWaves); the
+    // column W of this wave's rows (lane k: row wave + k WAVES); the
     // store waits for the loads above (they write the same LDS array)
-    const int r = wave + lane * kStripWaves;
+    const int r = wave + lane * WAVES;
     if (r < rows) tab[kStripGuard + r * kStripS + W] = make_float2(0.f, 0.f);
 }
 
@@ -141,12 +152,13 @@ __device__ __forceinline__ void stage_strip_table(float2* tab, __amdgpu_buffer_r
 // through the LDS path into a 256-byte array of their own, so they hold no
 // registers, and reads of the staged table need not wait for them (it is a
 // different LDS object); the next staging's waits cover them.
+template <int WAVES>
 __device__ __forceinline__ void prefetch_strip_table(float* sink, __amdgpu_buffer_rsrc_t rs,
         int tb, int W, int wave, int lane)
 {
     const unsigned b0 = (unsigned)((tb - kTablePadRows * W) * 8);
     const unsigned len = (unsigned)((W + 2 * kTablePadRows) * W * 8);
-    for (unsigned o = (unsigned)wave * 4096u; o < len; o += kStripWaves * 4096u)
+    for (unsigned o = (unsigned)wave * 4096u; o < len; o += WAVES * 4096u)
         __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void_t*)sink, 4,
                 o + (unsigned)lane * 64u, b0, 0, 0);
 }
@@ -217,9 +229,19 @@ __device__ __forceinline__ void mac_strip(float& re, float& im, const float2 c, 
         }                                                                     \
     }
 
-__global__ __launch_bounds__(kStripBlock) __attribute__((amdgpu_waves_per_eu(kStripWavesPerSimd)))
+template <class Cfg>
+__global__ __launch_bounds__(Cfg::kBlock) __attribute__((amdgpu_waves_per_eu(Cfg::kWaves / 4)))
 void k_grid_strips(const GridArgs<float> A)
 {
+    constexpr int kStripWaves = Cfg::kWaves;
+    constexpr int kStripBlock = Cfg::kBlock;
+    constexpr int kRegionW = Cfg::kRegionW;
+    constexpr int kRegionH = Cfg::kRegionH;
+    constexpr int kStripRows = Cfg::kRegionH;
+    constexpr int kStripSlots = Cfg::kSlots;
+    constexpr int kStripChunks = Cfg::kChunks;
+    constexpr int kDirectGroup = Cfg::kDirectGroup;
+    constexpr bool kComp = Cfg::kComp;
     __shared__ float2 tab[kStripLds];
     __shared__ float pf_sink[64];                 // prefetch loads land here
     const unsigned num_items = ld_const(A.d_num_items, 0);
@@ -290,6 +312,32 @@ void k_grid_strips(const GridArgs<float> A)
         float re[kStripSlots], im[kStripSlots];
         #pragma unroll
         for (int i = 0; i < kStripSlots; ++i) re[i] = im[i] = 0.f;
+        // Compensated: every kFlushEvery visits of the strip the FP32 sums
+        // re / im fold into TwoSum running sums (sre, sim) and error terms
+        // (cre, cim), which the write-out combines in FP64.
+        constexpr int kC = kComp ? kStripSlots : 1;
+        float sre[kC], sim[kC], cre[kC], cim[kC];
+        #pragma unroll
+        for (int i = 0; i < kC; ++i) sre[i] = sim[i] = cre[i] = cim[i] = 0.f;
+        int since = 0;                                 // visits since the last fold
+        auto visit_done = [&]() {
+            if constexpr (kComp)
+            {
+                if (++since == kFlushEvery)
+                {
+                    since = 0;
+                    #pragma unroll
+                    for (int i = 0; i < kStripSlots; ++i)
+                    {
+                        float s0, e0, s1, e1;
+                        two_sum(sre[i], re[i], s0, e0);
+                        two_sum(sim[i], im[i], s1, e1);
+                        sre[i] = s0; sim[i] = s1; cre[i] += e0; cim[i] += e1;
+                        re[i] = im[i] = 0.f;
+                    }
+                }
+            }
+        };
 
         // (resources based at the item's records keep buffer offsets small;
         // reads past the item's last visit return zero)
@@ -390,6 +438,7 @@ void k_grid_strips(const GridArgs<float> A)
                             mac_strip(re[2 * q + 1], im[2 * q + 1], tp[u][1], a);
                     }
                 }
+                visit_done();
             }
         }
         }
@@ -441,7 +490,7 @@ void k_grid_strips(const GridArgs<float> A)
                 if (rtb != staged_base)
                 {
                     __syncthreads();
-                    stage_strip_table(tab, rs_tab, rtb, 2 * meta_support(rz) + 1, wave, lane);
+                    stage_strip_table<kStripWaves>(tab, rs_tab, rtb, 2 * meta_support(rz) + 1, wave, lane);
                     __syncthreads();
                     staged_base = rtb;
                 }
@@ -449,7 +498,7 @@ void k_grid_strips(const GridArgs<float> A)
                 if (starts)           // the batch's next run: its table heads for L2
                 {
                     const int l1 = __builtin_ctzll(starts);
-                    prefetch_strip_table(pf_sink, rs_tab, bcast(mv.w, l1),
+                    prefetch_strip_table<kStripWaves>(pf_sink, rs_tab, bcast(mv.w, l1),
                                          2 * meta_support(bcast(mv.z, l1)) + 1, wave, lane);
                 }
 #endif
@@ -472,6 +521,7 @@ void k_grid_strips(const GridArgs<float> A)
                     const float2* t0 = (const float2*)((const char*)tab + o0);
                     const float2* t1 = (const float2*)((const char*)tab + o1);
                     GG_STRIP_PAIRS(cm)
+                    visit_done();
                 }
                 todo = seg & vneg;
                 while (todo)
@@ -488,6 +538,7 @@ void k_grid_strips(const GridArgs<float> A)
                     const float2* t0 = (const float2*)((const char*)tab + o0);
                     const float2* t1 = (const float2*)((const char*)tab + o1);
                     GG_STRIP_PAIRS_VMIRROR(cm)
+                    visit_done();
                 }
             }
         }
@@ -515,8 +566,16 @@ void k_grid_strips(const GridArgs<float> A)
                 {
                     float2* g = g0 + (size_t)i * step4;
                     float2 gv = *g;
-                    gv.x += re[i];
-                    gv.y += im[i];
+                    if constexpr (kComp)
+                    {
+                        gv.x = (float)((double)gv.x + ((double)sre[i] + (double)cre[i] + (double)re[i]));
+                        gv.y = (float)((double)gv.y + ((double)sim[i] + (double)cim[i] + (double)im[i]));
+                    }
+                    else
+                    {
+                        gv.x += re[i];
+                        gv.y += im[i];
+                    }
                     *g = gv;
                 }
             }
@@ -526,11 +585,25 @@ void k_grid_strips(const GridArgs<float> A)
             const unsigned slot = ld_const(A.slot_off, t) + m;
             unsigned lo = (unsigned)(lr * kRegionW + wave * kStripW + lc);
             __asm__ volatile("" : "+v"(lo));
-            float2* out = reinterpret_cast<float2*>(A.scratch) +
-                    (size_t)slot * (kRegionW * kRegionH) + lo;
-            #pragma unroll
-            for (int i = 0; i < kStripSlots; ++i)
-                out[i * 4 * kRegionW] = make_float2(re[i], im[i]);
+            if constexpr (kComp)
+            {
+                // (FP64 partials: the merge sums them in item order)
+                double2* out = reinterpret_cast<double2*>(A.scratch) +
+                        (size_t)slot * (kRegionW * kRegionH) + lo;
+                #pragma unroll
+                for (int i = 0; i < kStripSlots; ++i)
+                    out[i * 4 * kRegionW] = make_double2(
+                            (double)sre[i] + (double)cre[i] + (double)re[i],
+                            (double)sim[i] + (double)cim[i] + (double)im[i]);
+            }
+            else
+            {
+                float2* out = reinterpret_cast<float2*>(A.scratch) +
+                        (size_t)slot * (kRegionW * kRegionH) + lo;
+                #pragma unroll
+                for (int i = 0; i < kStripSlots; ++i)
+                    out[i * 4 * kRegionW] = make_float2(re[i], im[i]);
+            }
         }
         }
         // (the staged table stays valid for the next item: only staging

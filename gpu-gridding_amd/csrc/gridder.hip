@@ -493,16 +493,16 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     TRY((prepare_tables<TI, TA>(C, P, support, ov, csh, conv)));
     record(C, EV_PREP);
     const int half = (ov + 1) / 2, noff = half + 1;
-    // Gridding kernel and bucketing geometry: FP32 without compensation uses
-    // the strip kernel on kRegionW x kRegionH regions unless another kernel is asked
-    // for; the 64 x 64 kernels take every other case.
+    // Gridding kernel and bucketing geometry: FP32 uses the strip kernel on
+    // 256 x 128 regions (compensated: 192 x 64) unless another kernel is
+    // asked for; the 64 x 64 kernels take every other case.
     const bool comp = C->compensated;
-    const bool strips = std::is_same<T, float>::value && !comp && C->grid_kernel == 0;
+    const bool strips = std::is_same<T, float>::value && C->grid_kernel == 0;
     // (the direct 64 x 64 kernel launches one workgroup per item: not in the
     // host-sync-free mode, which takes the staged one instead)
     const bool direct64 = !strips && C->grid_kernel == 1 && !C->sync_free;
-    const int tw = strips ? kRegionW : kTileW;
-    const int th = strips ? kRegionH : kTileH;
+    const int tw = strips ? (comp ? StripComp::kRegionW : StripPlain::kRegionW) : kTileW;
+    const int th = strips ? (comp ? StripComp::kRegionH : StripPlain::kRegionH) : kTileH;
 
     // Jobs.
     C->h_jobs.resize(sizeof(JobDev<T>) * num_jobs);
@@ -837,7 +837,12 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
             const unsigned long long blk = 8ull * C->item_group;
             const unsigned wg8 = (unsigned)std::min<unsigned long long>(
                     std::max(8u, (unsigned)C->num_cu & ~7u), (num_items + blk - 1) / blk * blk);
-            hipLaunchKernelGGL(k_grid_strips, dim3(wg8), dim3(kStripBlock), 0, C->stream, ga);
+            if (comp)
+                hipLaunchKernelGGL(k_grid_strips<StripComp>, dim3(wg8), dim3(StripComp::kBlock),
+                        0, C->stream, ga);
+            else
+                hipLaunchKernelGGL(k_grid_strips<StripPlain>, dim3(wg8), dim3(StripPlain::kBlock),
+                        0, C->stream, ga);
             HIP_TRY(hipGetLastError());
         }
     }
