@@ -118,7 +118,7 @@ __device__ __forceinline__ X rb_x4(__amdgpu_buffer_rsrc_t rs, unsigned off)
 // The copy goes straight from memory to LDS (buffer loads with the LDS
 // flag): no registers hold the data, so every row's loads are issued at
 // once and a table costs one round trip to memory.  Wave w copies rows w,
-// w + kStripWaves, ...; one 16-byte load per lane and row moves columns
+// w + WAVES, ...; one 16-byte load per lane and row moves columns
 // 2l, 2l+1 to LDS (columns 0 .. 127 per row).  Lanes with 2l >= W load
 // nothing and write zeros; lane s (2s = W-1) also brings the tap after the
 // row's last (the next row's first), which one zero store per row then

@@ -99,7 +99,8 @@ MI355_EXPORT int mi355_ctx_set_stream(mi355_ctx* ctx, void* hip_stream);
 #define MI355_OPT_CACHE_KERNELS   4  /* 1: reuse prepared kernel tables while the
                                         conv_func pointer and shape are unchanged */
 #define MI355_OPT_GRID_KERNEL     5  /* 0: automatic (default): 16-column strips on
-                                        192x128 regions for plain FP32, else 2;
+                                        256x128 regions for FP32 (192x64 when
+                                        compensated), 2 for FP64;
                                         1: 64x64 tiles, tables read directly;
                                         2: 64x64 tiles, tables staged in LDS */
 #define MI355_OPT_MIN_STAGE_RUN   6  /* shortest same-table run staged in LDS (default 3) */
@@ -242,7 +243,7 @@ MI355_EXPORT int mi355_grid_wproj_f_batched_graph_dev(mi355_ctx* ctx,
 
 /* Autotuner for a batched workload: times the pipeline (into scratch
  * outputs; the job table's grids are not touched) for each gridding kernel
- * / tile geometry (16-column strips on 192x128 regions, or 64x64 tiles),
+ * / tile geometry (16-column strips on 256x128 regions, or 64x64 tiles),
  * work-item size and shortest staged run, and leaves the fastest settings
  * in the context's options (MI355_OPT_GRID_KERNEL, _VISITS_PER_ITEM,
  * _MIN_STAGE_RUN, _ITEM_GROUP).  *best_ms receives that setting's time per pass.  No
