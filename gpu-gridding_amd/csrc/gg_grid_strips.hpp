@@ -50,7 +50,8 @@ constexpr int kStripW = 16;                        // columns per wave
 //    pass alone took 53.7 ms against 60.5 ms, for a slightly slower direct
 //    pass.
 //  - Compensated FP32: 12 waves on 192 x 64 regions (16 slots per lane:
-//    32 running sums, and 64 TwoSum sums and error terms), 4 chunks.
+//    32 running sums, and 64 TwoSum sums and error terms), 8 chunks (2
+//    chunks: +5%, 4: +1%).
 #ifndef GG_STRIP_WAVES
 #define GG_STRIP_WAVES 16
 #endif
@@ -63,7 +64,13 @@ struct StripCfg
     static constexpr int kRegionH = ROWS;
     static constexpr int kSlots = ROWS / 4;          // slots of 16 columns x 4 rows
     static constexpr int kChunks = ROWS / 8;         // chunks of 8 rows
-    static constexpr int kDirectGroup = COMP ? 4 : (WAVES >= 16 ? 2 : 8);
+#ifndef GG_DIRECT_GROUP
+#define GG_DIRECT_GROUP (WAVES >= 16 ? 2 : 8)
+#endif
+#ifndef GG_DIRECT_GROUP_COMP
+#define GG_DIRECT_GROUP_COMP 8
+#endif
+    static constexpr int kDirectGroup = COMP ? GG_DIRECT_GROUP_COMP : GG_DIRECT_GROUP;
     static constexpr bool kComp = COMP;
     static_assert(ROWS % 16 == 0 && ROWS <= 128, "blocks of 16 rows, two LDS bases of 64 rows");
     static_assert(kRegionW * kRegionH % kMergeCells == 0, "merge blocks tile a region");
@@ -424,6 +431,7 @@ void k_grid_strips(const GridArgs<float> A)
                     for (int u = 0; u < kDirectGroup; ++u)
                     {
                         const int q = h + u;
+                        if (q >= kStripChunks) continue;
                         const unsigned so = (unsigned)(base + q * step);
                         if (sm & (1u << (2 * q))) tp[u][0] = load_tap(rs_tab, voff, so, (float2*)nullptr);
                         if (sm & (2u << (2 * q)))
@@ -433,6 +441,7 @@ void k_grid_strips(const GridArgs<float> A)
                     for (int u = 0; u < kDirectGroup; ++u)
                     {
                         const int q = h + u;
+                        if (q >= kStripChunks) continue;
                         if (sm & (1u << (2 * q))) mac_strip(re[2 * q], im[2 * q], tp[u][0], a);
                         if (sm & (2u << (2 * q)))
                             mac_strip(re[2 * q + 1], im[2 * q + 1], tp[u][1], a);
