@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from conftest import same_bits
-from mi355_gridding import synth
+from mi355_gridding import _lib, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -76,14 +76,24 @@ def _snap(jobs):
             for job in jobs]
 
 
-def test_graph_replays_equal_eager(gpu):
+@pytest.mark.parametrize("compensated", [0, 1])
+def test_graph_replays_equal_eager(gpu, compensated):
     """Graph bar: every replay, with eager work and host synchronisations in
     between, equals the eager (non-graph) batched pass bit for bit; so does
-    a replay after the inputs' values change in place."""
+    a replay after the inputs' values change in place.  Plain and
+    compensated accumulation (each its own strip kernel geometry)."""
     torch, ops = gpu
     ks, cat, jobs = _batched_case(torch, ops)
     args = (ks, jobs, cat["uu"], cat["vv"], cat["ww"], cat["vis"], cat["weight"])
     ctx = ops.context()
+    ctx.set_option(_lib.OPT_COMPENSATED, compensated)
+    try:
+        _graph_replays(torch, ops, ctx, args, cat, jobs)
+    finally:
+        ctx.set_option(_lib.OPT_COMPENSATED, 0)
+
+
+def _graph_replays(torch, ops, ctx, args, cat, jobs):
     _zero(jobs)
     ops.grid_batched(*args, ctx=ctx)
     torch.cuda.synchronize()
