@@ -251,3 +251,19 @@ def test_device_accumulates_across_calls(gpu):
     assert int(skipped.item()) == ts
     assert abs(norm.item() - (5.0 + 2 * tn)) <= 1e-12 * abs(2 * tn)
     assert errors(grid.cpu().numpy(), 2 * t)[0] <= 1e-5
+
+
+@pytest.mark.parametrize("G,smax", [(1001, 24), (130, 12), (300, 36)])
+def test_grid_sizes_off_the_region_grid(gpu, G, smax):
+    """Grid sizes that are no multiple of the strip regions (256 x 128, and
+    192 x 64 compensated) or smaller than one region: edge regions are cut
+    by the grid, and footprints that leave it are skipped (counted
+    bit-exactly).  Plain and compensated against the FP64 truth."""
+    torch, ops = gpu
+    c = small_config(0, 12_000, G, 32, smax, seed=5)
+    t, tn, ts = oracle.grid_fsel_d(c)
+    for mode in ("plain", "comp"):
+        g, n, s = run_grid(torch, ops, c, mode=mode)
+        assert s == ts and abs(n - tn) <= 1e-12 * abs(tn), mode
+        emax, el2 = errors(g, t)
+        assert emax <= 1e-5 and el2 <= 1e-5, (mode, emax, el2)
