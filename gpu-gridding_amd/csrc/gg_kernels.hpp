@@ -805,13 +805,8 @@ struct GridArgs
     const unsigned* run_start; const int* run_z; const int* run_tb;
     const unsigned long long* d_num_runs;
     unsigned min_stage_run;        // shortest run staged through LDS
-    // byte sizes for raw buffer access (strip kernel)
-    unsigned meta_bytes;           // meta (and amp) records, < 2^32
-    unsigned visit_bytes;          // visit_vis
     const int4* vmeta;             // meta / amp of each sorted visit (strip kernel)
     const typename V2<T>::t* vamp;        // weighted amplitude per sorted visit
-    unsigned vrec_bytes;           // unused
-    unsigned run_bytes;            // run_start / run_z / run_tb allocations
     const int* vlds;               // per sorted visit: its part of the staged LDS offsets
     unsigned item_group;           // strip kernel: consecutive items per XCD (>= 1)
     unsigned* item_ctr;            // strip kernel: 8 zeroed counters (positions

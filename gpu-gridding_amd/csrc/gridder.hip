@@ -806,15 +806,11 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     ga.run_tb = C->run_tb.as<int>();
     ga.d_num_runs = sc + SC_RUNS;
     ga.min_stage_run = C->min_stage_run;
-    ga.meta_bytes = (unsigned)std::min<unsigned long long>(n_total * 16ull, 0xFFFFFFFFull);
-    ga.visit_bytes = (unsigned)std::min<unsigned long long>(T_vis * 4ull, 0xFFFFFFFFull);
     if (strips && (T_vis + 1) * 4ull > 0xFFFFFFFFull)
         return fail(MI355_ERR_TOO_LARGE, "more than 2^30 tile visits in one call");
-    ga.run_bytes = (unsigned)std::min<unsigned long long>((T_vis + 1) * 4ull, 0xFFFFFFFFull);
     ga.vmeta = C->vmeta.as<int4>();
     ga.vamp = C->vamp.as<T2>();
     ga.vlds = C->vlds.as<int>();
-    ga.vrec_bytes = 0;   // (the strip kernel sizes its record windows per item)
     ga.item_group = C->item_group;
     // the strip kernel's persistent grid takes positions from per-XCD
     // counters (zeroed with the other scalars at the start of the pass)
