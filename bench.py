@@ -223,6 +223,46 @@ def bench_weights(case, d, ctx, no_cpu, reps=5):
     return out
 
 
+def bench_differentiable(case, d, reps=3):
+    """Forward (gridder) and backward (degridding adjoint) of the
+    differentiable gridder, loss 0.5 ||grid - target||^2 with a seeded
+    complex target, gradients for the amplitudes and the weights."""
+    import numpy as np
+    import torch
+    from mi355_gridding import torch_ops as ops
+    G = case["grid_size"]
+    ks = ops.KernelStack(case["support"], case["oversample"], case["conv_size_half"],
+                         case["conv_func"])
+    gen = torch.Generator(device="cuda").manual_seed(1234)
+    target = torch.randn((G, G, 2), generator=gen, device="cuda")
+    vis = d["vis"].reshape(-1, 2).clone().requires_grad_(True)
+    wt = d["weight"].clone().requires_grad_(True)
+    args = (ks, d["uu"], d["vv"], d["ww"], case["cell_size_rad"], case["w_scale"], G)
+    fw, bw = [], []
+    for r in range(reps + 1):
+        vis.grad = None
+        wt.grad = None
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
+        g, _ = ops.grid_autograd(vis, wt, *args)
+        loss = 0.5 * ((g - target) ** 2).sum()
+        e1.record()
+        loss.backward()
+        e2.record()
+        torch.cuda.synchronize()
+        if r:
+            fw.append(e0.elapsed_time(e1))
+            bw.append(e1.elapsed_time(e2))
+    N = case["num_points"]
+    out = dict(forward_ms=round(float(np.mean(fw)), 3), backward_ms=round(float(np.mean(bw)), 3),
+               value=round(N / ((np.mean(fw) + np.mean(bw)) * 1e-3) / 1e6, 3),
+               unit="Mvis/s (forward + backward)",
+               note="torch_ops.GridWProj: forward = gridder plus the loss; backward = "
+                    "degridding kernel for d/dvis and d/dweight")
+    log("differentiable %s" % json.dumps(out))
+    return out
+
+
 def bench_image(grid, G, norm, ctx, no_cpu, reps=5):
     """Grid -> image on the device over the gridded grid; numpy's FFT of a
     4096^2 grid on the host as the CPU figure."""
@@ -419,6 +459,7 @@ def main():
                            "host<->device inside the call (best of 2)")
         log("drop-in %s" % json.dumps(dropin))
 
+    differentiable = bench_differentiable(case, d) if a.config == 4 else None
     weights = None if a.no_weights else bench_weights(case, d, ctx, a.no_cpu_baseline)
     image = None if a.no_image else bench_image(grid, G, float(norm.item()), ctx,
                                                 a.no_cpu_baseline)
@@ -467,6 +508,7 @@ def main():
         "accuracy": accuracy,
         "cpu_baseline": cpu,
         "dropin": dropin,
+        "differentiable": differentiable,
         "weights": weights,
         "image": image,
     }
