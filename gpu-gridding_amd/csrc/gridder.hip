@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -115,6 +116,61 @@ void release(DevBuf& b)
     b.cap = 0;
 }
 
+// Pinned host memory (hipHostMalloc) that the host-pointer drop-ins stage
+// through: the DMA engines read and write it at full PCIe rate, which
+// pageable memory does not reach.
+struct HostBuf
+{
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+int ensure_host(HostBuf& b, size_t bytes)
+{
+    if (bytes <= b.cap) return MI355_OK;
+    if (b.p) { (void)hipHostFree(b.p); b.p = nullptr; b.cap = 0; }
+    if (hipHostMalloc(&b.p, bytes, hipHostMallocDefault) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        b.p = nullptr;
+        return MI355_ERR_OUT_OF_MEMORY;
+    }
+    b.cap = bytes;
+    return MI355_OK;
+}
+
+void release_host(HostBuf& b)
+{
+    if (b.p) (void)hipHostFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+// rows x row_bytes from src (row pitch sp) to dst (row pitch dp), the rows
+// split over up to 16 host threads (one per 4 MiB)
+void copy_rows(void* dst, size_t dp, const void* src, size_t sp, size_t row_bytes, size_t rows)
+{
+    const size_t total = row_bytes * rows;
+    unsigned nt = std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    nt = (unsigned)std::min<size_t>(nt, std::max<size_t>(1, total >> 22));
+    nt = (unsigned)std::min<size_t>(nt, rows);
+    auto part = [&](size_t r0, size_t r1) {
+        for (size_t r = r0; r < r1; ++r)
+            std::memcpy(static_cast<char*>(dst) + r * dp,
+                        static_cast<const char*>(src) + r * sp, row_bytes);
+    };
+    if (nt <= 1) { part(0, rows); return; }
+    std::vector<std::thread> th;
+    const size_t per = (rows + nt - 1) / nt;
+    for (unsigned t = 1; t < nt; ++t)
+    {
+        const size_t r0 = std::min(rows, t * per), r1 = std::min(rows, r0 + per);
+        if (r0 < r1) th.emplace_back(part, r0, r1);
+    }
+    part(0, std::min(rows, per));
+    for (auto& t : th) t.join();
+}
+
 int bits_for(unsigned long long n)   // bits needed to hold values 0..n-1
 {
     int b = 0;
@@ -187,6 +243,10 @@ struct mi355_ctx
     // drop-in staging
     DevBuf in_conv, in_uu, in_vv, in_ww, in_vis, in_weight, in_grid, in_norm, in_skipped;
     DevBuf in_weight_out;
+    // pinned staging of the host-pointer drop-ins: two bands in turn, each
+    // with the event of the copy that last used it
+    HostBuf h_stage[2];
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
     // grid -> image
     DevBuf fft_work, in_image;
     hipfftHandle fft_plan = 0;
@@ -965,6 +1025,93 @@ int select_dev(mi355_ctx* C, size_t P, const int* support, int ov, size_t N,
 // ---------------------------------------------------------------------------
 // Host-pointer drop-in.
 // ---------------------------------------------------------------------------
+// Transfers between the caller's (pageable) arrays and the device go
+// through the context's two pinned bands of kStageBand bytes: the host
+// threads copy band k while the DMA engine moves band k-1.
+constexpr size_t kStageBand = 64u << 20;
+
+int stage_slots(mi355_ctx* C, size_t bytes)
+{
+    for (int i = 0; i < 2; ++i)
+    {
+        if (C->stage_ev[i]) HIP_TRY(hipEventSynchronize(C->stage_ev[i]));
+        else HIP_TRY(hipEventCreateWithFlags(&C->stage_ev[i], hipEventDisableTiming));
+        TRY(ensure_host(C->h_stage[i], bytes));
+    }
+    return MI355_OK;
+}
+
+// rows x row_bytes, host (pitch hp) -> device (pitch dp), on C->stream
+int stage_upload(mi355_ctx* C, void* dev, size_t dp, const void* host, size_t hp,
+        size_t row_bytes, size_t rows)
+{
+    if (!rows || !row_bytes) return MI355_OK;
+    const size_t band = std::max<size_t>(1, kStageBand / row_bytes);
+    TRY(stage_slots(C, std::min(rows, band) * row_bytes));
+    for (size_t k = 0, r0 = 0; r0 < rows; ++k, r0 += band)
+    {
+        const int sl = (int)(k & 1);
+        const size_t n = std::min(band, rows - r0);
+        HIP_TRY(hipEventSynchronize(C->stage_ev[sl]));     // its last DMA is done
+        copy_rows(C->h_stage[sl].p, row_bytes, static_cast<const char*>(host) + r0 * hp, hp,
+                  row_bytes, n);
+        HIP_TRY(hipMemcpy2DAsync(static_cast<char*>(dev) + r0 * dp, dp, C->h_stage[sl].p,
+                row_bytes, row_bytes, n, hipMemcpyHostToDevice, C->stream));
+        HIP_TRY(hipEventRecord(C->stage_ev[sl], C->stream));
+    }
+    return MI355_OK;
+}
+
+int stage_upload_1d(mi355_ctx* C, void* dev, const void* host, size_t bytes)
+{
+    const size_t full = bytes / kStageBand, tail = bytes - full * kStageBand;
+    TRY(stage_upload(C, dev, kStageBand, host, kStageBand, kStageBand, full));
+    return stage_upload(C, static_cast<char*>(dev) + full * kStageBand, tail,
+                        static_cast<const char*>(host) + full * kStageBand, tail, tail, 1);
+}
+
+// rows x row_bytes, device (pitch dp) -> host (pitch hp); returns when the
+// host has them.  (Before band k is read into slot k&1, the host has copied
+// band k-2 out of it.)
+int stage_download(mi355_ctx* C, void* host, size_t hp, const void* dev, size_t dp,
+        size_t row_bytes, size_t rows)
+{
+    if (!rows || !row_bytes) return MI355_OK;
+    const size_t band = std::max<size_t>(1, kStageBand / row_bytes);
+    const size_t nb = (rows + band - 1) / band;
+    TRY(stage_slots(C, std::min(rows, band) * row_bytes));
+    for (size_t k = 0; k <= nb; ++k)
+    {
+        if (k < nb)
+        {
+            const int sl = (int)(k & 1);
+            const size_t r0 = k * band, n = std::min(band, rows - r0);
+            HIP_TRY(hipMemcpy2DAsync(C->h_stage[sl].p, row_bytes,
+                    static_cast<const char*>(dev) + r0 * dp, dp, row_bytes, n,
+                    hipMemcpyDeviceToHost, C->stream));
+            HIP_TRY(hipEventRecord(C->stage_ev[sl], C->stream));
+        }
+        if (k >= 1)
+        {
+            const size_t j = k - 1;
+            const int sl = (int)(j & 1);
+            const size_t r0 = j * band, n = std::min(band, rows - r0);
+            HIP_TRY(hipEventSynchronize(C->stage_ev[sl]));
+            copy_rows(static_cast<char*>(host) + r0 * hp, hp, C->h_stage[sl].p, row_bytes,
+                      row_bytes, n);
+        }
+    }
+    return MI355_OK;
+}
+
+int stage_download_1d(mi355_ctx* C, void* host, const void* dev, size_t bytes)
+{
+    const size_t full = bytes / kStageBand, tail = bytes - full * kStageBand;
+    TRY(stage_download(C, host, kStageBand, dev, kStageBand, kStageBand, full));
+    return stage_download(C, static_cast<char*>(host) + full * kStageBand, tail,
+                          static_cast<const char*>(dev) + full * kStageBand, tail, tail, 1);
+}
+
 template <typename T>
 struct UploadWindow : BeforeGrid
 {
@@ -977,11 +1124,9 @@ struct UploadWindow : BeforeGrid
         typedef typename V2<T>::t T2;
         const size_t pitch = (size_t)G * sizeof(T2);
         const size_t off = (size_t)box.y * G + box.x;
-        HIP_TRY(hipMemcpy2DAsync(C->in_grid.as<T2>() + off, pitch,
+        return stage_upload(C, C->in_grid.as<T2>() + off, pitch,
                 reinterpret_cast<const T2*>(host_grid) + off, pitch,
-                (size_t)(box.z - box.x + 1) * sizeof(T2), (size_t)(box.w - box.y + 1),
-                hipMemcpyHostToDevice, C->stream));
-        return MI355_OK;
+                (size_t)(box.z - box.x + 1) * sizeof(T2), (size_t)(box.w - box.y + 1));
     }
 };
 
@@ -1012,12 +1157,12 @@ int grid_host(size_t P, const int* support, int ov, int csh, const T* conv,
     TRY(ensure(C->in_grid, (size_t)G * G * sizeof(T2)));
     TRY(ensure(C->in_norm, sizeof(double)));
     TRY(ensure(C->in_skipped, sizeof(unsigned long long)));
-    HIP_TRY(hipMemcpyAsync(C->in_conv.p, conv, conv_n * sizeof(T), hipMemcpyHostToDevice, C->stream));
-    HIP_TRY(hipMemcpyAsync(C->in_uu.p, uu, N * sizeof(T), hipMemcpyHostToDevice, C->stream));
-    HIP_TRY(hipMemcpyAsync(C->in_vv.p, vv, N * sizeof(T), hipMemcpyHostToDevice, C->stream));
-    HIP_TRY(hipMemcpyAsync(C->in_ww.p, ww, N * sizeof(T), hipMemcpyHostToDevice, C->stream));
-    HIP_TRY(hipMemcpyAsync(C->in_vis.p, vis, 2 * N * sizeof(T), hipMemcpyHostToDevice, C->stream));
-    HIP_TRY(hipMemcpyAsync(C->in_weight.p, weight, N * sizeof(T), hipMemcpyHostToDevice, C->stream));
+    TRY(stage_upload_1d(C, C->in_conv.p, conv, conv_n * sizeof(T)));
+    TRY(stage_upload_1d(C, C->in_uu.p, uu, N * sizeof(T)));
+    TRY(stage_upload_1d(C, C->in_vv.p, vv, N * sizeof(T)));
+    TRY(stage_upload_1d(C, C->in_ww.p, ww, N * sizeof(T)));
+    TRY(stage_upload_1d(C, C->in_vis.p, vis, 2 * N * sizeof(T)));
+    TRY(stage_upload_1d(C, C->in_weight.p, weight, N * sizeof(T)));
     const double zero = 0.0;
     HIP_TRY(hipMemcpyAsync(C->in_norm.p, &zero, sizeof(double), hipMemcpyHostToDevice, C->stream));
     // The host conv_func may change between calls at the same address.
@@ -1033,9 +1178,9 @@ int grid_host(size_t P, const int* support, int ov, int csh, const T* conv,
     {
         const size_t pitch = (size_t)G * sizeof(T2);
         const size_t off = (size_t)up.box.y * G + up.box.x;
-        HIP_TRY(hipMemcpy2DAsync(reinterpret_cast<T2*>(grid) + off, pitch,
+        TRY(stage_download(C, reinterpret_cast<T2*>(grid) + off, pitch,
                 C->in_grid.as<T2>() + off, pitch, (size_t)(up.box.z - up.box.x + 1) * sizeof(T2),
-                (size_t)(up.box.w - up.box.y + 1), hipMemcpyDeviceToHost, C->stream));
+                (size_t)(up.box.w - up.box.y + 1)));
     }
     double dnorm = 0.0;
     unsigned long long dskip = 0;
@@ -1246,9 +1391,8 @@ int upload_weight_window(mi355_ctx* C, const T* grid, int G, const int* box)
 {
     const size_t ww = (size_t)(box[2] - box[0] + 1), wh = (size_t)(box[3] - box[1] + 1);
     TRY(ensure(C->in_grid, ww * wh * sizeof(T)));
-    HIP_TRY(hipMemcpy2DAsync(C->in_grid.p, ww * sizeof(T), grid + (size_t)box[1] * G + box[0],
-            (size_t)G * sizeof(T), ww * sizeof(T), wh, hipMemcpyHostToDevice, C->stream));
-    return MI355_OK;
+    return stage_upload(C, C->in_grid.p, ww * sizeof(T), grid + (size_t)box[1] * G + box[0],
+            (size_t)G * sizeof(T), ww * sizeof(T), wh);
 }
 
 template <typename T>
@@ -1268,9 +1412,9 @@ int weights_write_host(size_t N, const T* uu, const T* vv, const T* weight, T ce
     TRY(ensure(C->in_weight, N * sizeof(T)));
     TRY(ensure(C->in_skipped, sizeof(unsigned long long)));
     TRY(ensure(C->scalars, SC_COUNT * sizeof(unsigned long long)));
-    HIP_TRY(hipMemcpyAsync(C->in_uu.p, uu, N * sizeof(T), hipMemcpyHostToDevice, C->stream));
-    HIP_TRY(hipMemcpyAsync(C->in_vv.p, vv, N * sizeof(T), hipMemcpyHostToDevice, C->stream));
-    HIP_TRY(hipMemcpyAsync(C->in_weight.p, weight, N * sizeof(T), hipMemcpyHostToDevice, C->stream));
+    TRY(stage_upload_1d(C, C->in_uu.p, uu, N * sizeof(T)));
+    TRY(stage_upload_1d(C, C->in_vv.p, vv, N * sizeof(T)));
+    TRY(stage_upload_1d(C, C->in_weight.p, weight, N * sizeof(T)));
     unsigned long long* sc = C->scalars.as<unsigned long long>();
     int* box = reinterpret_cast<int*>(sc + SC_W_BOX);
     unsigned long long* skipped = C->in_skipped.as<unsigned long long>();
@@ -1295,10 +1439,8 @@ int weights_write_host(size_t N, const T* uu, const T* vv, const T* weight, T ce
                 (unsigned)((unsigned long long)G * G), G, C->in_grid.as<T>(), hbox[0], hbox[1],
                 (unsigned long long)ww);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpy2DAsync(grid + (size_t)hbox[1] * G + hbox[0], (size_t)G * sizeof(T),
-                C->in_grid.p, ww * sizeof(T), ww * sizeof(T), wh, hipMemcpyDeviceToHost,
-                C->stream));
-        HIP_TRY(hipStreamSynchronize(C->stream));
+        TRY(stage_download(C, grid + (size_t)hbox[1] * G + hbox[0], (size_t)G * sizeof(T),
+                C->in_grid.p, ww * sizeof(T), ww * sizeof(T), wh));
     }
     *num_skipped = (size_t)hskip;
     return ok();
@@ -1323,8 +1465,8 @@ int weights_read_host(size_t N, const T* uu, const T* vv, const T* weight_in, T*
     TRY(ensure(C->in_weight_out, N * sizeof(T)));
     TRY(ensure(C->in_skipped, sizeof(unsigned long long)));
     TRY(ensure(C->scalars, SC_COUNT * sizeof(unsigned long long)));
-    HIP_TRY(hipMemcpyAsync(C->in_uu.p, uu, N * sizeof(T), hipMemcpyHostToDevice, C->stream));
-    HIP_TRY(hipMemcpyAsync(C->in_vv.p, vv, N * sizeof(T), hipMemcpyHostToDevice, C->stream));
+    TRY(stage_upload_1d(C, C->in_uu.p, uu, N * sizeof(T)));
+    TRY(stage_upload_1d(C, C->in_vv.p, vv, N * sizeof(T)));
     unsigned long long* sc = C->scalars.as<unsigned long long>();
     int* box = reinterpret_cast<int*>(sc + SC_W_BOX);
     unsigned long long* skipped = C->in_skipped.as<unsigned long long>();
@@ -1342,12 +1484,9 @@ int weights_read_host(size_t N, const T* uu, const T* vv, const T* weight_in, T*
     HIP_TRY(hipStreamSynchronize(C->stream));
     if (hbox[0] <= hbox[2])
     {
-        HIP_TRY(hipMemcpyAsync(C->in_weight.p, weight_in, N * sizeof(T), hipMemcpyHostToDevice,
-                C->stream));
+        TRY(stage_upload_1d(C, C->in_weight.p, weight_in, N * sizeof(T)));
         // points off the grid leave their weight_out entries as they were
-        if (hskip)
-            HIP_TRY(hipMemcpyAsync(C->in_weight_out.p, weight_out, N * sizeof(T),
-                    hipMemcpyHostToDevice, C->stream));
+        if (hskip) TRY(stage_upload_1d(C, C->in_weight_out.p, weight_out, N * sizeof(T)));
         TRY(upload_weight_window<T>(C, grid, G, hbox));
         const size_t ww = (size_t)(hbox[2] - hbox[0] + 1);
         hipLaunchKernelGGL(k_weight_read<T>, dim3(grid_blocks(N, 256, 16384)), dim3(256), 0,
@@ -1355,9 +1494,7 @@ int weights_read_host(size_t N, const T* uu, const T* vv, const T* weight_in, T*
                 C->in_weight_out.as<T>(), (unsigned long long)N, gs, G, C->in_grid.as<T>(),
                 hbox[0], hbox[1], (unsigned long long)ww, (unsigned long long*)nullptr);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(weight_out, C->in_weight_out.p, N * sizeof(T),
-                hipMemcpyDeviceToHost, C->stream));
-        HIP_TRY(hipStreamSynchronize(C->stream));
+        TRY(stage_download_1d(C, weight_out, C->in_weight_out.p, N * sizeof(T)));
     }
     *num_skipped = (size_t)hskip;
     return ok();
@@ -1430,12 +1567,9 @@ int image_host(int G, const T* grid, double norm, T* image)
     const size_t n = (size_t)G * G;
     TRY(ensure(C->in_grid, 2 * n * sizeof(T)));
     TRY(ensure(C->in_image, n * sizeof(T)));
-    HIP_TRY(hipMemcpyAsync(C->in_grid.p, grid, 2 * n * sizeof(T), hipMemcpyHostToDevice,
-            C->stream));
+    TRY(stage_upload_1d(C, C->in_grid.p, grid, 2 * n * sizeof(T)));
     TRY(image_dev<T>(C, G, C->in_grid.as<T>(), norm, C->in_image.as<T>()));
-    HIP_TRY(hipMemcpyAsync(image, C->in_image.p, n * sizeof(T), hipMemcpyDeviceToHost,
-            C->stream));
-    HIP_TRY(hipStreamSynchronize(C->stream));
+    TRY(stage_download_1d(C, image, C->in_image.p, n * sizeof(T)));
     return ok();
 }
 
@@ -1519,9 +1653,14 @@ MI355_EXPORT int mi355_ctx_destroy(mi355_ctx* C)
         &C->jobs_d, &C->job_chunk_off_d, &C->job_vis_off_d, &C->job_tile_base_d,
         &C->job_out_d, &C->job_bbox_d, &C->in_conv, &C->in_uu, &C->in_vv, &C->in_ww,
         &C->in_vis, &C->in_weight, &C->in_grid, &C->in_norm, &C->in_skipped, &C->in_weight_out,
-        &C->fft_work, &C->in_image};
+        &C->fft_work, &C->in_image, &C->vmeta, &C->vamp, &C->vlds};
     if (C->fft_valid) (void)hipfftDestroy(C->fft_plan);
     for (DevBuf* b : all) release(*b);
+    for (int i = 0; i < 2; ++i)
+    {
+        release_host(C->h_stage[i]);
+        if (C->stage_ev[i]) (void)hipEventDestroy(C->stage_ev[i]);
+    }
     for (int i = 0; i < EV_COUNT; ++i) if (C->ev[i]) (void)hipEventDestroy(C->ev[i]);
     delete C;
     return ok();
