@@ -233,6 +233,19 @@ def test_deterministic_and_item_split_invariant(gpu):
     c2, _, _ = run_grid(torch, ops, c, mode="comp", visits_per_item=37)
     assert same_bits(c1, run_grid(torch, ops, c, mode="comp")[0])
     assert errors(c2, c1)[0] <= 1e-6
+    # which workgroup takes which item (XCD groups, counters) does not touch
+    # the result: each item sums on its own, partials merge in item order
+    ctx = ops.context()
+    g0 = ctx.get_option(_lib.OPT_ITEM_GROUP)
+    try:
+        for grp in (8, 3):
+            ctx.set_option(_lib.OPT_ITEM_GROUP, grp)
+            gg, ng, _ = run_grid(torch, ops, c, ctx=ctx)
+            assert same_bits(gg, g1) and ng == n1, grp
+            gg, _, _ = run_grid(torch, ops, c, ctx=ctx, visits_per_item=37)
+            assert same_bits(gg, g3), grp
+    finally:
+        ctx.set_option(_lib.OPT_ITEM_GROUP, g0)
 
 
 def test_device_accumulates_across_calls(gpu):
