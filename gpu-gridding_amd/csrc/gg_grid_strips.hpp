@@ -363,13 +363,15 @@ void k_grid_strips(const GridArgs<float> A)
         };
         // Per visit (scalar, from lane l of the batch): support, footprint
         // origin relative to the strip (kc: footprint column of strip column
-        // 0; top: footprint row 0), chunks q0..q1 touched, coefficients.  (A
-        // visit of the region's list meets its rows: top <= 127, top+2s >= 0.)
+        // 0; top: footprint row 0, region row), the mask cm of the 8-row
+        // chunks touched (both precomputed in the record: y = (cm << 16) |
+        // (top & 0xffff)), coefficients.  (A visit of the region's list meets
+        // its rows: top <= region height - 1, top + 2s >= 0.)
 // The FMA coefficients (re += c.x a.x + c.y a.y, im += c.x a.z + c.y a.w)
 // come from the weighted amplitude (vr, vi) and the conjugation flag:
 // (vr, -vi, vi, vr), or (vr, vi, vi, -vr) when conjugated.
 #define GG_VISIT_HEAD                                                           \
-        const int x = bcast(mv.x, l), y = bcast(mv.y, l), z = bcast(mv.z, l);   \
+        const int x = bcast(mv.x, l), py = bcast(mv.y, l), z = bcast(mv.z, l);  \
         float4 a;                                                               \
         {                                                                       \
             const float vr = bcast(av.x, l), vi = bcast(av.y, l);               \
@@ -378,10 +380,8 @@ void k_grid_strips(const GridArgs<float> A)
         }                                                                       \
         const int s = meta_support(z);                                          \
         const int kc = xs - (x - s);                                            \
-        const int top = y - s - ys;                                             \
-        const int q0 = max(top, 0) >> 3;                                        \
-        const int q1 = min(top + 2 * s, kStripRows - 1) >> 3;                   \
-        const int cm = (int)((2u << q1) - (1u << q0));
+        const int top = (py << 16) >> 16;                                       \
+        const int cm = (int)((unsigned)py >> 16);
 
         {
         GG_LANE_RC

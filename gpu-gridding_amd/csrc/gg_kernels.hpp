@@ -721,12 +721,16 @@ __global__ void k_run_starts(const unsigned* __restrict__ flag,
 // visits in the strip kernel instead of an index load and a dependent one),
 // with kVisitStaged set where the visit's run is staged in LDS (at least
 // min_run visits and a table at most max_w wide) and kVisitRunStart where
-// the visit starts a run.
+// the visit starts a run; y holds the visit's placement in its region (see
+// below) and vlds the visit's part of its staged LDS offsets.
 template <typename T>
 __global__ void k_gather_visits(const unsigned* __restrict__ vals,
         const int4* __restrict__ meta, const typename V4<T>::t* __restrict__ amp,
         const unsigned* __restrict__ run_flag, const unsigned* __restrict__ run_excl,
         const unsigned* __restrict__ run_start, unsigned min_run, int max_w, int lds_stride,
+        const unsigned* __restrict__ keys, int key_table_bits,
+        const JobDev<T>* __restrict__ jobs, const unsigned* __restrict__ job_tile_base,
+        int num_jobs, int region_h,
         const unsigned long long* __restrict__ d_n, int4* __restrict__ vmeta,
         typename V2<T>::t* __restrict__ vamp, int* __restrict__ vlds)
 {
@@ -740,7 +744,6 @@ __global__ void k_gather_visits(const unsigned* __restrict__ vals,
         if (run_start[r + 1] - run_start[r] >= min_run && 2 * meta_support(m.z) + 1 <= max_w)
             m.z |= kVisitStaged;
         if (run_flag[p]) m.z |= kVisitRunStart;
-        vmeta[p] = m;
         const typename V4<T>::t am = amp[i];
         vamp[p] = {am.x, am.z};            // the weighted amplitude (re, im)
         // The visit's part of its staged-tap LDS byte offsets (the strip
@@ -752,6 +755,20 @@ __global__ void k_gather_visits(const unsigned* __restrict__ vals,
         const int cu = (m.z & kFlagNegU) ? m.x + sp : sp - m.x;
         const int rv = (m.z & kFlagNegV) ? m.y + sp : sp - m.y;
         vlds[p] = 8 * (cu + rv * lds_stride);
+        // The visit's rows in its region (sorted key -> tile -> job -> row
+        // of the region): footprint row 0 at region row top, and the mask
+        // of the 8-row chunks the footprint touches, packed into y as
+        // (chunk mask << 16) | (top & 0xffff).
+        const unsigned t = keys[p] >> key_table_bits;
+        const int j = find_job(job_tile_base, num_jobs, t);
+        const unsigned local = t - jobs[j].tile_base;
+        const int ys = (int)(local / (unsigned)jobs[j].ntx) * region_h;
+        const int top = m.y - sp - ys;
+        const int q0 = max(top, 0) >> 3;
+        const int q1 = min(top + 2 * sp, region_h - 1) >> 3;
+        const unsigned cm = (2u << q1) - (1u << q0);
+        m.y = (int)((cm << 16) | ((unsigned)top & 0xffffu));
+        vmeta[p] = m;
     }
 }
 

@@ -787,8 +787,10 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
         TRY(ensure(C->vlds, T_vis * sizeof(int)));
         hipLaunchKernelGGL(k_gather_visits<T>, dim3(grid_blocks(T_vis, 256, 65535)), dim3(256), 0,
                 C->stream, svals, C->meta.as<int4>(), C->amp.as<T4>(), run_flag, run_excl,
-                C->run_start.as<unsigned>(), C->min_stage_run, kStripMaxW, kStripS, sc + SC_VISITS,
-                C->vmeta.as<int4>(), C->vamp.as<T2>(), C->vlds.as<int>());
+                C->run_start.as<unsigned>(), C->min_stage_run, kStripMaxW, kStripS, skeys,
+                key_table_bits, C->jobs_d.as<JobDev<T>>(), C->job_tile_base_d.as<unsigned>(),
+                num_jobs, th, sc + SC_VISITS, C->vmeta.as<int4>(), C->vamp.as<T2>(),
+                C->vlds.as<int>());
         HIP_TRY(hipGetLastError());
     }
 
