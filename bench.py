@@ -329,7 +329,7 @@ def main():
                     help="only run plain device copies of MIB MiB, then exit")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py: this build measures one GPU only (the gridder has no multi-GPU "
+        print("bench.py: measures one GPU only (the gridder has no multi-GPU "
               "path); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
