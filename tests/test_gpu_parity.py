@@ -280,3 +280,30 @@ def test_grid_sizes_off_the_region_grid(gpu, G, smax):
         assert s == ts and abs(n - tn) <= 1e-12 * abs(tn), mode
         emax, el2 = errors(g, t)
         assert emax <= 1e-5 and el2 <= 1e-5, (mode, emax, el2)
+
+
+@pytest.mark.parametrize("smax,N", [(100, 4000), (160, 3000)])
+def test_large_supports_against_fp64_oracle(gpu, smax, N):
+    """Supports beyond the shapes above (16 planes, 4 .. smax): footprints up
+    to 201 and 321 wide -- wider than any staged table (only direct reads in
+    the strip kernel), taller than a 128-row region and, at 321, wider than
+    a 256-column one, so a visit is bucketed into several regions in both
+    directions.  Every kernel (strips plain and compensated, 64 x 64 direct
+    and staged) and the FP32-selection / FP64 path against the FP64 truth;
+    num_skipped bit-exact, norm to 1e-12."""
+    torch, ops = gpu
+    c = small_config(0, N, 1024, 16, smax, seed=11)
+    assert int(np.max(c["support"])) == smax
+    t, tn, ts = oracle.grid_fsel_d(c)
+    ctx = ops.context()
+    try:
+        for kernel, mode in ((0, "plain"), (0, "comp"), (1, "plain"), (2, "plain")):
+            ctx.set_option(_lib.OPT_GRID_KERNEL, kernel)
+            g, n, s = run_grid(torch, ops, c, mode=mode, ctx=ctx)
+            assert s == ts and abs(n - tn) <= 1e-12 * abs(tn), (kernel, mode, s, ts, n, tn)
+            emax, el2 = errors(g, t)
+            assert emax <= 1e-5 and el2 <= 1e-5, (kernel, mode, emax, el2)
+    finally:
+        ctx.set_option(_lib.OPT_GRID_KERNEL, 0)
+    gd, nd, sd = run_grid(torch, ops, c, mode="fsel_d")
+    assert errors(gd, t)[0] <= 1e-12 and sd == ts and abs(nd - tn) <= 1e-12 * abs(tn)
