@@ -20,11 +20,14 @@ def to_dev(torch, c, dtype):
             for k in ("uu", "vv", "ww", "vis", "weight")}
 
 
+# (support 100: footprints up to 201 wide, several grid regions in each
+# direction, tables read only directly by the gridder)
+@pytest.mark.parametrize("smax,N,P", [(24, 60_000, 64), (100, 6_000, 16)])
 @pytest.mark.parametrize("prec,tol", [("f", 1e-5), ("d", 1e-12)])
-def test_dot_product_adjoint(gpu, prec, tol):
+def test_dot_product_adjoint(gpu, prec, tol, smax, N, P):
     torch, ops = gpu
     dtype = torch.float32 if prec == "f" else torch.float64
-    c = synth.make(4, num_points=60_000, grid_size=1024, num_w_planes=64, max_support=24)
+    c = synth.make(4, num_points=N, grid_size=1024, num_w_planes=P, max_support=smax)
     ks = ops.KernelStack(c["support"], c["oversample"], c["conv_size_half"], c["conv_func"],
                          dtype=dtype)
     d = to_dev(torch, c, dtype)
