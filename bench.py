@@ -16,6 +16,9 @@ gather); `peak` is MI355X's FP32 figure, 157.3 TFLOP/s, which needs packed
 FMAs, and `frac_nonpacked` prices against the 78.6 TFLOP/s of one FMA per
 lane per clock.  `traffic` is the kernel's HBM bytes per launch from the
 FETCH_SIZE / WRITE_SIZE counter summary in profiles/, when present.
+`accuracy` compares one pass's FP32 grid (plain and compensated) with the
+FP64-accumulation truth and times the compensated form, the truth and the
+FP64 gridder (FP64 selection and accumulation) on the same inputs.
 `cpu_baseline` times the reference's serial oskar_grid_wproj_f (compiled
 from its sources into oracle/_ref, else the oracle's restatement) on one
 core on a prefix of the same visibilities.  `dropin` (with --dropin) times
@@ -409,7 +412,13 @@ def main():
         ops.grid(*step_args, g32, n32, skipped, ctx=ctx)
         g64 = torch.zeros((G, G, 2), dtype=torch.float64, device="cuda")
         n64 = torch.zeros_like(norm)
+        torch.cuda.synchronize()
+        f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        f0.record(stream)
         ops.grid_fsel_d(*step_args, g64, n64, skipped, ctx=ctx)
+        f1.record(stream)
+        torch.cuda.synchronize()
+        fsel_d_ms = f0.elapsed_time(f1)
         ctx.set_option(_lib.OPT_COMPENSATED, 1)
         gc = torch.zeros_like(grid)
         ops.grid(*step_args, gc, torch.zeros_like(norm), skipped, ctx=ctx)
@@ -433,8 +442,27 @@ def main():
             dd = g.double() - g64
             return dict(max_rel=dd.abs().max().item() / m64,
                         l2_rel=torch.linalg.norm(dd).item() / l64)
+        # the FP64 gridder (oskar_grid_wproj_d's arithmetic: FP64 selection
+        # and accumulation) on the same inputs: one warm pass, one timed
+        ks64 = ops.KernelStack(case["support"], case["oversample"], case["conv_size_half"],
+                               case["conv_func"], dtype=torch.float64)
+        args64 = (ks64,) + tuple(d[k].double() for k in ("uu", "vv", "ww", "vis", "weight")) + \
+            (case["cell_size_rad"], case["w_scale"], G)
+        gd = torch.zeros((G, G, 2), dtype=torch.float64, device="cuda")
+        nd = torch.zeros_like(norm)
+        ops.grid(*args64, gd, nd, skipped, ctx=ctx)
+        torch.cuda.synchronize()
+        f0.record(stream)
+        ops.grid(*args64, gd, nd, skipped, ctx=ctx)
+        f1.record(stream)
+        torch.cuda.synchronize()
+        fp64_ms = f0.elapsed_time(f1)
+        del gd, args64, ks64
         accuracy = dict(plain_fp32=err(g32), compensated_fp32=err(gc),
                         compensated_ms_per_step=round(comp_ms, 3),
+                        fp64_accumulation_ms=round(fsel_d_ms, 3),
+                        fp64_gridder_ms=round(fp64_ms, 3),
+                        fp64_gridder_mvis_s=round(N / fp64_ms / 1e3, 2),
                         norm_rel=abs(n32.item() - n64.item()) / abs(n64.item()),
                         truth="FP64 accumulation, FP32 selection (mi355_grid_wproj_fsel_d_dev)")
         del g64, gc, g32
