@@ -23,7 +23,8 @@ FP64 gridder (FP64 selection and accumulation) on the same inputs.
 from its sources into oracle/_ref, else the oracle's restatement) on one
 core on a prefix of the same visibilities.  `dropin` (with --dropin) times
 the host-pointer drop-in mi355_grid_wproj_f, host-device copies included.
-`image` times the grid -> image transform (centred inverse 2-D FFT of the
+`driver` (with --driver) times main.cpp's flow over .vis / FITS files of the
+2.15M-visibility shape.  `image` times the grid -> image transform (centred inverse 2-D FFT of the
 gridded 16384^2 grid, real part / norm) beside numpy's FFT of a 4096^2
 grid on one core.  `weights` times the uniform-weighting pre-pass on the same points (device
 write of the 16384^2 weights grid, then the read of every point's weight),
@@ -305,6 +306,50 @@ def bench_image(grid, G, norm, ctx, no_cpu, reps=5):
     return out
 
 
+def bench_driver():
+    """main.cpp's flow on the GPU (mi355_grid_vis_file: read the blocks of a
+    .vis file, uniform weights, grid every block, FP32) over files of the
+    2.15M-visibility / 714-plane / support-36 / 8192^2 shape (208 stations x
+    100 times, 10 times per block) written to a scratch directory; the
+    coordinates are written in metres at a frequency equal to c, so they
+    read back as the same wavelengths."""
+    import shutil
+    import tempfile
+    import numpy as np
+    from mi355_gridding import synth, vis_io
+    c = synth.make(1)
+    S, T, tpb = 208, 100, 10
+    B = S * (S - 1) // 2
+    assert B * T == c["num_points"]
+    tmp = tempfile.mkdtemp(prefix="mi355_driver_")
+    root = os.path.join(tmp, "synthetic")
+    try:
+        kw = (c["support"], c["oversample"], c["grid_size"], float(c["cell_size_rad"]),
+              float(c["w_scale"]))
+        vis_io.write_kernel(root + "_KERNELS_REAL.fits", c["conv_func"][..., 0], *kw)
+        vis_io.write_kernel(root + "_KERNELS_IMAG.fits", c["conv_func"][..., 1], *kw)
+        vis = (c["vis"][:, 0] + 1j * c["vis"][:, 1]).reshape(T, B)
+        vis_io.write_vis(root + ".vis", c["uu"].reshape(T, B), c["vv"].reshape(T, B),
+                         c["ww"].reshape(T, B), vis, S, tpb, 299792458.0)
+        vis_io.grid_vis_file(root, want_grid=False)            # first call: device set-up
+        runs = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            _, res = vis_io.grid_vis_file(root, want_grid=False)
+            runs.append((time.perf_counter() - t0, res))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    wall, res = min(runs, key=lambda r: r[0])
+    return dict(seconds_total=round(wall, 4), seconds_read=round(res["seconds_read"], 4),
+                seconds_weights=round(res["seconds_weights"], 4),
+                seconds_grid=round(res["seconds_grid"], 4), num_blocks=res["num_blocks"],
+                num_points=res["num_points"],
+                mvis_s_grid=round(res["num_points"] / res["seconds_grid"] / 1e6, 2),
+                mvis_s_total=round(res["num_points"] / wall / 1e6, 2),
+                note="mi355_grid_vis_file over a .vis file and FITS kernel cubes in a scratch "
+                     "directory (file reads, weights and gridding of every block; best of 2)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -324,6 +369,9 @@ def main():
                     help="strip kernel: consecutive work items dealt to one XCD")
     ap.add_argument("--dropin", action="store_true",
                     help="also time the host-pointer drop-in (PCIe included)")
+    ap.add_argument("--driver", action="store_true",
+                    help="also time main.cpp's flow over .vis / FITS files of the "
+                         "2.15M-visibility shape")
     ap.add_argument("--no-weights", action="store_true",
                     help="skip the uniform-weighting pre-pass leg")
     ap.add_argument("--no-image", action="store_true",
@@ -487,6 +535,11 @@ def main():
                            "host<->device inside the call (best of 2)")
         log("drop-in %s" % json.dumps(dropin))
 
+    driver = None
+    if a.driver:
+        driver = bench_driver()
+        log("driver %s" % json.dumps(driver))
+
     differentiable = bench_differentiable(case, d) if a.config == 4 else None
     weights = None if a.no_weights else bench_weights(case, d, ctx, a.no_cpu_baseline)
     image = None if a.no_image else bench_image(grid, G, float(norm.item()), ctx,
@@ -536,6 +589,7 @@ def main():
         "accuracy": accuracy,
         "cpu_baseline": cpu,
         "dropin": dropin,
+        "driver": driver,
         "differentiable": differentiable,
         "weights": weights,
         "image": image,
