@@ -55,8 +55,44 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(case, seconds):
-    """Serial gridder on one core over a prefix of the workload."""
+def host_facts():
+    """CPU model and core counts of the host the bench runs on."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return dict(cpu_model=model, nproc=os.cpu_count(), nproc_usable=usable)
+
+
+def grid_errors(test, truth):
+    """max |G - T|, max |G - T| / max |T| and ||G - T||_2 / ||T||_2 over the
+    grids' real and imaginary parts (numpy or torch arrays, any precision)."""
+    import numpy as np
+    t = np.asarray(truth, np.float64)
+    dd = np.asarray(test, np.float64) - t
+    mt, lt = float(np.max(np.abs(t))), float(np.linalg.norm(t))
+    ma = float(np.max(np.abs(dd)))
+    return dict(max_abs=ma, max_rel=ma / mt if mt else ma,
+                l2_rel=float(np.linalg.norm(dd)) / lt if lt else float(np.linalg.norm(dd)))
+
+
+def cpu_baseline(case, seconds, d=None, ks=None, ctx=None):
+    """Serial gridder on one core over a prefix of the workload (about
+    `seconds` of CPU work), and on the same prefix the accuracy statement:
+    the serial FP64 gridder (oskar_grid_wproj_d's arithmetic) and the FP64
+    accumulation with FP32 selection as truths, the serial FP32 gridder's own
+    error against them (the reference's FP32 error) and, given the device
+    inputs d, the GPU's plain and compensated FP32 grids of the same prefix
+    against them."""
     import numpy as np
     from oracle import oracle
     use_ref = oracle.ref() is not None
@@ -64,23 +100,64 @@ def cpu_baseline(case, seconds):
     G = case["grid_size"]
     grid = np.zeros((G, G, 2), np.float32)
 
-    def run(n):
-        sub = dict(case)
+    def sub(n):
+        s = dict(case)
         for k in ("uu", "vv", "ww", "weight"):
-            sub[k] = case[k][:n]
-        sub["vis"] = case["vis"][:n]
-        sub["num_points"] = n
+            s[k] = case[k][:n]
+        s["vis"] = case["vis"][:n]
+        s["num_points"] = n
+        return s
+
+    def run(n):
+        grid[...] = 0
         t0 = time.perf_counter()
-        oracle.grid_f(sub, use_ref=use_ref, grid=grid)
-        return time.perf_counter() - t0
+        _, nrm, _ = oracle.grid_f(sub(n), use_ref=use_ref, grid=grid)
+        return time.perf_counter() - t0, nrm
 
     n0 = 20_000
-    t0 = run(n0)
+    t0, _ = run(n0)
     n = int(min(case["num_points"], max(n0, n0 * seconds / max(t0, 1e-6))))
-    t = run(n)
-    return dict(value=n / t / 1e6, unit="Mvis/s", cores=1, kind=kind,
-                sample="first %d of %d visibilities (%.1f s), serial oskar_grid_wproj_f, "
-                       "%s" % (n, case["num_points"], t, platform.processor() or "x86-64"))
+    t, norm_f = run(n)
+    facts = host_facts()
+    out = dict(value=n / t / 1e6, unit="Mvis/s", cores=1, kind=kind,
+               sample="first %d of %d visibilities (%.1f s), serial oskar_grid_wproj_f on one "
+                      "core of %d (%s)" % (n, case["num_points"], t, facts["nproc"],
+                                           facts["cpu_model"]),
+               host=facts)
+    # accuracy on the same prefix
+    s = sub(n)
+    g_d, norm_d, _ = oracle.grid_d(s)                 # FP64 selection and accumulation
+    g_t, norm_t, _ = oracle.grid_fsel_d(s)            # FP32 selection, FP64 accumulation
+    acc = dict(prefix_points=n,
+               serial_f_vs_d=grid_errors(grid, g_d),
+               serial_f_vs_fsel_d=grid_errors(grid, g_t),
+               serial_norm_rel_vs_d=abs(norm_f - norm_d) / abs(norm_d))
+    if d is not None:
+        import torch
+        from mi355_gridding import _lib, torch_ops as ops
+        comp0 = ctx.get_option(_lib.OPT_COMPENSATED)
+        for name, comp in (("gpu_plain", 0), ("gpu_compensated", 1)):
+            gg = torch.zeros((G, G, 2), dtype=torch.float32, device="cuda")
+            nn = torch.zeros(1, dtype=torch.float64, device="cuda")
+            sk = torch.zeros(1, dtype=torch.int64, device="cuda")
+            ctx.set_option(_lib.OPT_COMPENSATED, comp)
+            ops.grid(ks, d["uu"][:n], d["vv"][:n], d["ww"][:n], d["vis"][:2 * n]
+                     if d["vis"].dim() == 1 else d["vis"][:n], d["weight"][:n],
+                     case["cell_size_rad"], case["w_scale"], G, gg, nn, sk, ctx=ctx)
+            torch.cuda.synchronize()
+            h = gg.cpu().numpy()
+            del gg
+            acc[name + "_vs_fsel_d"] = grid_errors(h, g_t)
+            acc[name + "_vs_d"] = grid_errors(h, g_d)
+            acc[name + "_norm_rel_vs_serial_f"] = abs(nn.item() - norm_f) / abs(norm_f)
+        ctx.set_option(_lib.OPT_COMPENSATED, comp0)
+    acc["note"] = ("errors over the real and imaginary parts of the grid of the prefix; d = the "
+                   "serial FP64 gridder (FP64 selection, oskar_grid_wproj_d), fsel_d = FP32 "
+                   "selection with FP64 accumulation (the truth with the FP32 gridder's own "
+                   "cells and kernel indices)")
+    out["accuracy"] = acc
+    del g_d, g_t
+    return out
 
 
 def bench_batched(a):
@@ -488,8 +565,8 @@ def main():
 
         def err(g):
             dd = g.double() - g64
-            return dict(max_rel=dd.abs().max().item() / m64,
-                        l2_rel=torch.linalg.norm(dd).item() / l64)
+            ma = dd.abs().max().item()
+            return dict(max_abs=ma, max_rel=ma / m64, l2_rel=torch.linalg.norm(dd).item() / l64)
         # the FP64 gridder (oskar_grid_wproj_d's arithmetic: FP64 selection
         # and accumulation) on the same inputs: one warm pass, one timed
         ks64 = ops.KernelStack(case["support"], case["oversample"], case["conv_size_half"],
@@ -512,7 +589,10 @@ def main():
                         fp64_gridder_ms=round(fp64_ms, 3),
                         fp64_gridder_mvis_s=round(N / fp64_ms / 1e3, 2),
                         norm_rel=abs(n32.item() - n64.item()) / abs(n64.item()),
-                        truth="FP64 accumulation, FP32 selection (mi355_grid_wproj_fsel_d_dev)")
+                        truth="FP64 accumulation, FP32 selection (mi355_grid_wproj_fsel_d_dev, "
+                              "checked against the serial oracle at small sizes and on tiles); "
+                              "cpu_baseline.accuracy states the same errors against the serial "
+                              "FP64 oracle on a prefix, beside the serial FP32 gridder's own")
         del g64, gc, g32
         log("accuracy %s" % json.dumps(accuracy))
 
@@ -554,7 +634,7 @@ def main():
 
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline(case, a.cpu_seconds)
+        cpu = cpu_baseline(case, a.cpu_seconds, d, ks, ctx)
         log("cpu baseline %s" % json.dumps(cpu))
 
     out = {
@@ -576,7 +656,12 @@ def main():
                    "accumulation": "compensated" if a.compensated else "plain",
                    "tile_kernel": a.kernel, "visits_per_item": a.visits_per_item,
                    "min_stage_run": a.min_stage_run, "item_group": a.item_group},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3),
+        "roofline": {"bound": "mfma",
+                     "bound_note": "the kernel's binding unit is the FP32 vector FMA pipe (no "
+                                   "matrix instructions: the tap lookup is an indexed gather); "
+                                   "the line's two classes are hbm and mfma, so the "
+                                   "compute-bound class is named",
+                     "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "frac_nonpacked": round(achieved / PEAK_FP32_NONPACKED_TFLOPS, 4),

@@ -78,43 +78,6 @@ struct DevBuf
     template <typename X> X* as() const { return static_cast<X*>(p); }
 };
 
-int ensure(DevBuf& b, size_t bytes)
-{
-    if (bytes == 0) bytes = 256;
-    if (bytes <= b.cap) return MI355_OK;
-    if (b.p)
-    {
-        // Work already queued may still read the old buffer.
-        (void)hipDeviceSynchronize();
-        (void)hipFree(b.p);
-        b.p = nullptr;
-        b.cap = 0;
-    }
-    const size_t want = bytes + bytes / 8;      // headroom for slowly growing calls
-    hipError_t e = hipMalloc(&b.p, want);
-    if (e != hipSuccess)
-    {
-        (void)hipGetLastError();
-        e = hipMalloc(&b.p, bytes);
-        if (e != hipSuccess)
-        {
-            (void)hipGetLastError();
-            b.p = nullptr;
-            return fail(MI355_ERR_OUT_OF_MEMORY, "device allocation of %zu bytes failed", bytes);
-        }
-        b.cap = bytes;
-        return MI355_OK;
-    }
-    b.cap = want;
-    return MI355_OK;
-}
-
-void release(DevBuf& b)
-{
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.cap = 0;
-}
 
 // Pinned host memory (hipHostMalloc) that the host-pointer drop-ins stage
 // through: the DMA engines read and write it at full PCIe rate, which
@@ -219,6 +182,11 @@ struct mi355_ctx
     hipStream_t capture_stream = nullptr;
     std::vector<unsigned char> graph_key;
     bool stats_untimed = false;      // last call was a graph launch (no phase events)
+    // The batched graph runs on a context of its own, whose buffers, job
+    // table and uploads no other call touches; the last call on this
+    // context was a graph call when stats_in_graph is set.
+    mi355_ctx* graph_ctx = nullptr;
+    bool stats_in_graph = false;
 
     // prepared kernel tables
     const void* cache_conv = nullptr;
@@ -262,12 +230,83 @@ struct mi355_ctx
 
     hipEvent_t ev[EV_COUNT] = {};
     mi355_stats stats = {};
+
+    // device workspace held by the context's buffers, and its cap (0: none)
+    size_t ws_bytes = 0;
+    unsigned long long max_workspace = 0;
+    // set by mi355_ctx_set_stream when the stream changes: switch_ev marks
+    // the end of the work queued on the previous stream
+    hipEvent_t switch_ev = nullptr;
+    bool switch_pending = false;
 };
 
 namespace {
 
 std::mutex g_default_lock;
 mi355_ctx* g_default[64] = {};
+
+// Work this context has queued that may still read its buffers: everything
+// on its current stream, and on the stream it used before the last
+// mi355_ctx_set_stream (an event recorded there at the switch).
+int wait_ctx_work(mi355_ctx* C)
+{
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    if (C->switch_pending)
+    {
+        HIP_TRY(hipEventSynchronize(C->switch_ev));
+        C->switch_pending = false;
+    }
+    return MI355_OK;
+}
+
+// Grow-only device allocation of a context's buffer b to at least `bytes`.
+// A buffer that grows is freed only after the context's queued work is done
+// (not the whole device's).  The context's workspace cap
+// (MI355_OPT_MAX_WORKSPACE) is checked before any allocation.
+int ensure(mi355_ctx* C, DevBuf& b, size_t bytes)
+{
+    if (bytes == 0) bytes = 256;
+    if (bytes <= b.cap) return MI355_OK;
+    const size_t others = C->ws_bytes - b.cap;
+    if (C->max_workspace && others + bytes > C->max_workspace)
+        return fail(MI355_ERR_OUT_OF_MEMORY,
+                "workspace of %zu bytes would exceed the context's cap of %zu bytes",
+                others + bytes, (size_t)C->max_workspace);
+    if (b.p)
+    {
+        TRY(wait_ctx_work(C));
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        C->ws_bytes -= b.cap;
+        b.cap = 0;
+    }
+    size_t want = bytes + bytes / 8;      // headroom for slowly growing calls
+    if (C->max_workspace && others + want > C->max_workspace) want = bytes;
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess)
+    {
+        (void)hipGetLastError();
+        want = bytes;
+        e = hipMalloc(&b.p, bytes);
+        if (e != hipSuccess)
+        {
+            (void)hipGetLastError();
+            b.p = nullptr;
+            return fail(MI355_ERR_OUT_OF_MEMORY, "device allocation of %zu bytes failed", bytes);
+        }
+    }
+    b.cap = want;
+    C->ws_bytes += want;
+    return MI355_OK;
+}
+
+void release(mi355_ctx* C, DevBuf& b)
+{
+    if (b.p) (void)hipFree(b.p);
+    C->ws_bytes -= b.cap;
+    b.p = nullptr;
+    b.cap = 0;
+}
 
 void record(mi355_ctx* C, Phase p)
 {
@@ -291,6 +330,7 @@ int init_ctx(mi355_ctx* C, int device)
             ncu > 0)
         C->num_cu = ncu;
     for (int i = 0; i < EV_COUNT; ++i) HIP_TRY(hipEventCreate(&C->ev[i]));
+    HIP_TRY(hipEventCreateWithFlags(&C->switch_ev, hipEventDisableTiming));
     return MI355_OK;
 }
 
@@ -387,7 +427,7 @@ int prepare_tables(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     const unsigned long long tail = std::max(guard, 2ull * kMaxSupport + 1);
     if ((off + tail) * sizeof(T2) > 0x7FFFFFFFull)
         return fail(MI355_ERR_TOO_LARGE, "expanded kernel tables exceed 2 GiB");
-    TRY(ensure(C->tables, (off + tail) * sizeof(T2)));
+    TRY(ensure(C, C->tables, (off + tail) * sizeof(T2)));
     C->table_tail = tail;
     {
         // zero the guard and the tail (fill kernels: no memset nodes)
@@ -398,9 +438,9 @@ int prepare_tables(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
                 C->stream, reinterpret_cast<unsigned*>(C->tables.as<T2>() + off), tail * wpe, 0u);
         HIP_TRY(hipGetLastError());
     }
-    TRY(ensure(C->norm_table, P * (2 * half + 1) * (2 * half + 1) * sizeof(double)));
-    TRY(ensure(C->support_d, P * sizeof(int)));
-    TRY(ensure(C->plane_off_d, P * sizeof(unsigned)));
+    TRY(ensure(C, C->norm_table, P * (2 * half + 1) * (2 * half + 1) * sizeof(double)));
+    TRY(ensure(C, C->support_d, P * sizeof(int)));
+    TRY(ensure(C, C->plane_off_d, P * sizeof(unsigned)));
     C->cache_support.assign(support, support + P);
     if (!C->capturing)   // (a captured pipeline reuses what the eager run uploaded)
     {
@@ -438,7 +478,7 @@ int scan_u32(mi355_ctx* C, const unsigned* in, unsigned* out, DevCount cnt,
         int write_end)
 {
     const unsigned long long nchunk = (n_bound + kScanChunk - 1) / kScanChunk;
-    TRY(ensure(C->scan_partial, (nchunk + 1) * sizeof(unsigned long long)));
+    TRY(ensure(C, C->scan_partial, (nchunk + 1) * sizeof(unsigned long long)));
     unsigned long long* part = C->scan_partial.as<unsigned long long>();
     const unsigned blocks = grid_blocks(nchunk, 1, 16384);
     hipLaunchKernelGGL(k_scan_reduce, dim3(blocks), dim3(kScanThreads), 0, C->stream, in, cnt, part);
@@ -467,8 +507,8 @@ int radix_sort(mi355_ctx* C, unsigned long long n_bound, const unsigned long lon
     const int bits = (key_bits + 8) / 9 < (key_bits + 7) / 8 ? 9 : 8;
     const unsigned radix = 1u << bits;
     const unsigned long long nchunk = (n_bound + kSortChunk - 1) / kSortChunk;
-    TRY(ensure(C->hist, radix * nchunk * sizeof(unsigned)));
-    TRY(ensure(C->hist_off, (radix * nchunk + 1) * sizeof(unsigned)));
+    TRY(ensure(C, C->hist, radix * nchunk * sizeof(unsigned)));
+    TRY(ensure(C, C->hist_off, (radix * nchunk + 1) * sizeof(unsigned)));
     int cur = 0;
     for (int shift = 0; shift < key_bits; shift += bits)
     {
@@ -549,6 +589,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     typedef typename V2<T>::t T2;
     typedef typename V4<T>::t T4;
     C->stats = mi355_stats();
+    C->stats_in_graph = false;
     record(C, EV_START);
     TRY((prepare_tables<TI, TA>(C, P, support, ov, csh, conv)));
     record(C, EV_PREP);
@@ -626,12 +667,12 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     const int key_table_bits = std::min(kappa_bits, 32 - tile_bits);
     const int key_table_shift = kappa_bits - key_table_bits;
 
-    TRY(ensure(C->jobs_d, C->h_jobs.size()));
-    TRY(ensure(C->job_chunk_off_d, (num_jobs + 1) * sizeof(unsigned)));
-    TRY(ensure(C->job_tile_base_d, (num_jobs + 1) * sizeof(unsigned)));
-    TRY(ensure(C->job_vis_off_d, (num_jobs + 1) * sizeof(unsigned long long)));
-    TRY(ensure(C->job_out_d, 3 * num_jobs * sizeof(unsigned long long)));
-    TRY(ensure(C->job_bbox_d, num_jobs * sizeof(int4)));
+    TRY(ensure(C, C->jobs_d, C->h_jobs.size()));
+    TRY(ensure(C, C->job_chunk_off_d, (num_jobs + 1) * sizeof(unsigned)));
+    TRY(ensure(C, C->job_tile_base_d, (num_jobs + 1) * sizeof(unsigned)));
+    TRY(ensure(C, C->job_vis_off_d, (num_jobs + 1) * sizeof(unsigned long long)));
+    TRY(ensure(C, C->job_out_d, 3 * num_jobs * sizeof(unsigned long long)));
+    TRY(ensure(C, C->job_bbox_d, num_jobs * sizeof(int4)));
     if (!C->capturing)
     {
         HIP_TRY(hipMemcpyAsync(C->jobs_d.p, C->h_jobs.data(), C->h_jobs.size(), hipMemcpyHostToDevice, C->stream));
@@ -641,21 +682,21 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     }
 
     // Selection.
-    TRY(ensure(C->meta, n_total * sizeof(int4)));
-    TRY(ensure(C->amp, n_total * sizeof(T4)));
-    TRY(ensure(C->num_visits, n_total * sizeof(unsigned)));
-    TRY(ensure(C->visit_off, (n_total + 1) * sizeof(unsigned)));
-    TRY(ensure(C->part_norm, chunks * sizeof(double)));
-    TRY(ensure(C->part_skipped, chunks * sizeof(unsigned long long)));
-    TRY(ensure(C->part_taps, chunks * sizeof(unsigned long long)));
-    TRY(ensure(C->part_nonfinite, chunks * sizeof(unsigned long long)));
-    TRY(ensure(C->part_bbox, chunks * sizeof(int4)));
-    TRY(ensure(C->scalars, SC_COUNT * sizeof(unsigned long long)));
+    TRY(ensure(C, C->meta, n_total * sizeof(int4)));
+    TRY(ensure(C, C->amp, n_total * sizeof(T4)));
+    TRY(ensure(C, C->num_visits, n_total * sizeof(unsigned)));
+    TRY(ensure(C, C->visit_off, (n_total + 1) * sizeof(unsigned)));
+    TRY(ensure(C, C->part_norm, chunks * sizeof(double)));
+    TRY(ensure(C, C->part_skipped, chunks * sizeof(unsigned long long)));
+    TRY(ensure(C, C->part_taps, chunks * sizeof(unsigned long long)));
+    TRY(ensure(C, C->part_nonfinite, chunks * sizeof(unsigned long long)));
+    TRY(ensure(C, C->part_bbox, chunks * sizeof(int4)));
+    TRY(ensure(C, C->scalars, SC_COUNT * sizeof(unsigned long long)));
     unsigned long long* sc = C->scalars.as<unsigned long long>();
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, C->stream,
             reinterpret_cast<unsigned*>(sc), (unsigned long long)(2 * SC_COUNT), 0u);
 
-    SelectArgs<TI, TA> sa;
+    SelectArgs<TI, TA> sa{};
     sa.uu = uu; sa.vv = vv; sa.ww = ww; sa.vis = vis; sa.weight = weight;
     sa.jobs = C->jobs_d.as<JobDev<T>>(); sa.num_jobs = num_jobs;
     sa.job_chunk_off = C->job_chunk_off_d.as<unsigned>();
@@ -741,8 +782,8 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     // Emit (tile | table) keys and sort them.
     for (int b = 0; b < 2; ++b)
     {
-        TRY(ensure(C->keys[b], T_vis * sizeof(unsigned)));
-        TRY(ensure(C->vals[b], T_vis * sizeof(unsigned)));
+        TRY(ensure(C, C->keys[b], T_vis * sizeof(unsigned)));
+        TRY(ensure(C, C->vals[b], T_vis * sizeof(unsigned)));
     }
     hipLaunchKernelGGL(k_emit<T>, dim3(grid_blocks(n_total, 256, 16384)), dim3(256), 0, C->stream,
             C->meta.as<int4>(), C->num_visits.as<unsigned>(), C->visit_off.as<unsigned>(),
@@ -772,9 +813,9 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
         DevCount rc{sc + SC_VISITS, 1, 1, 0};
         TRY(scan_u32(C, run_flag, run_excl, rc, T_vis, sc + SC_RUNS,
                 reinterpret_cast<unsigned*>(sc + SC_OVERFLOW), 0));
-        TRY(ensure(C->run_start, (T_vis + 1) * sizeof(unsigned)));
-        TRY(ensure(C->run_z, (T_vis + 1) * sizeof(int)));
-        TRY(ensure(C->run_tb, (T_vis + 1) * sizeof(int)));
+        TRY(ensure(C, C->run_start, (T_vis + 1) * sizeof(unsigned)));
+        TRY(ensure(C, C->run_z, (T_vis + 1) * sizeof(int)));
+        TRY(ensure(C, C->run_tb, (T_vis + 1) * sizeof(int)));
         hipLaunchKernelGGL(k_run_starts, dim3(vb), dim3(256), 0, C->stream, run_flag, run_excl,
                 svals, C->meta.as<int4>(), sc + SC_VISITS, sc + SC_RUNS,
                 C->run_start.as<unsigned>(), C->run_z.as<int>(), C->run_tb.as<int>());
@@ -782,9 +823,9 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     }
     if (strips)
     {
-        TRY(ensure(C->vmeta, T_vis * sizeof(int4)));
-        TRY(ensure(C->vamp, T_vis * sizeof(T2)));
-        TRY(ensure(C->vlds, T_vis * sizeof(int)));
+        TRY(ensure(C, C->vmeta, T_vis * sizeof(int4)));
+        TRY(ensure(C, C->vamp, T_vis * sizeof(T2)));
+        TRY(ensure(C, C->vlds, T_vis * sizeof(int)));
         hipLaunchKernelGGL(k_gather_visits<T>, dim3(grid_blocks(T_vis, 256, 65535)), dim3(256), 0,
                 C->stream, svals, C->meta.as<int4>(), C->amp.as<T4>(), run_flag, run_excl,
                 C->run_start.as<unsigned>(), C->min_stage_run, kStripMaxW, kStripS, skeys,
@@ -795,12 +836,12 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     }
 
     // Tiles -> work items.
-    TRY(ensure(C->tile_start, num_tiles * sizeof(unsigned)));
-    TRY(ensure(C->tile_end, num_tiles * sizeof(unsigned)));
-    TRY(ensure(C->n_items, num_tiles * sizeof(unsigned)));
-    TRY(ensure(C->n_slots, num_tiles * sizeof(unsigned)));
-    TRY(ensure(C->item_off, (num_tiles + 1) * sizeof(unsigned)));
-    TRY(ensure(C->slot_off, (num_tiles + 1) * sizeof(unsigned)));
+    TRY(ensure(C, C->tile_start, num_tiles * sizeof(unsigned)));
+    TRY(ensure(C, C->tile_end, num_tiles * sizeof(unsigned)));
+    TRY(ensure(C, C->n_items, num_tiles * sizeof(unsigned)));
+    TRY(ensure(C, C->n_slots, num_tiles * sizeof(unsigned)));
+    TRY(ensure(C, C->item_off, (num_tiles + 1) * sizeof(unsigned)));
+    TRY(ensure(C, C->slot_off, (num_tiles + 1) * sizeof(unsigned)));
     const unsigned tb = grid_blocks(num_tiles, 256, 16384);
     hipLaunchKernelGGL(k_fill_u32, dim3(tb), dim3(256), 0, C->stream, C->tile_start.as<unsigned>(), (unsigned long long)num_tiles, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(tb), dim3(256), 0, C->stream, C->tile_end.as<unsigned>(), (unsigned long long)num_tiles, 0u);
@@ -833,7 +874,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
         num_items = host_sc[SC_ITEMS];
         num_slots = host_sc[SC_SLOTS];
     }
-    TRY(ensure(C->item_tile, num_items * sizeof(unsigned)));
+    TRY(ensure(C, C->item_tile, num_items * sizeof(unsigned)));
     hipLaunchKernelGGL(k_item_map, dim3(tb), dim3(256), 0, C->stream, C->item_off.as<unsigned>(),
             num_tiles, C->item_tile.as<unsigned>());
     HIP_TRY(hipGetLastError());
@@ -842,7 +883,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
 
     // Grid.
     const size_t part_bytes = comp ? sizeof(double2) : sizeof(T2);
-    TRY(ensure(C->scratch, num_slots * (unsigned long long)(tw * th) * part_bytes));
+    TRY(ensure(C, C->scratch, num_slots * (unsigned long long)(tw * th) * part_bytes));
     GridArgs<T> ga;
     ga.tables = C->tables.as<T2>();
     ga.table_bytes = (unsigned)((C->table_elems + C->table_tail) * sizeof(T2));
@@ -1010,7 +1051,7 @@ int select_dev(mi355_ctx* C, size_t P, const int* support, int ov, size_t N,
     if (N && (!uu || !vv || !ww || !out)) return fail(MI355_ERR_INVALID_ARG, "NULL pointer argument");
     if (N == 0) return ok();
     HIP_TRY(hipSetDevice(C->device));
-    TRY(ensure(C->support_d, P * sizeof(int)));
+    TRY(ensure(C, C->support_d, P * sizeof(int)));
     C->cache_support.assign(support, support + P);
     C->tables_valid = false;
     HIP_TRY(hipMemcpyAsync(C->support_d.p, C->cache_support.data(), P * sizeof(int),
@@ -1150,15 +1191,15 @@ int grid_host(size_t P, const int* support, int ov, int csh, const T* conv,
     HIP_TRY(hipSetDevice(C->device));
     typedef typename V2<T>::t T2;
     const size_t conv_n = (size_t)P * csh * csh * 2;
-    TRY(ensure(C->in_conv, conv_n * sizeof(T)));
-    TRY(ensure(C->in_uu, N * sizeof(T)));
-    TRY(ensure(C->in_vv, N * sizeof(T)));
-    TRY(ensure(C->in_ww, N * sizeof(T)));
-    TRY(ensure(C->in_vis, 2 * N * sizeof(T)));
-    TRY(ensure(C->in_weight, N * sizeof(T)));
-    TRY(ensure(C->in_grid, (size_t)G * G * sizeof(T2)));
-    TRY(ensure(C->in_norm, sizeof(double)));
-    TRY(ensure(C->in_skipped, sizeof(unsigned long long)));
+    TRY(ensure(C, C->in_conv, conv_n * sizeof(T)));
+    TRY(ensure(C, C->in_uu, N * sizeof(T)));
+    TRY(ensure(C, C->in_vv, N * sizeof(T)));
+    TRY(ensure(C, C->in_ww, N * sizeof(T)));
+    TRY(ensure(C, C->in_vis, 2 * N * sizeof(T)));
+    TRY(ensure(C, C->in_weight, N * sizeof(T)));
+    TRY(ensure(C, C->in_grid, (size_t)G * G * sizeof(T2)));
+    TRY(ensure(C, C->in_norm, sizeof(double)));
+    TRY(ensure(C, C->in_skipped, sizeof(unsigned long long)));
     TRY(stage_upload_1d(C, C->in_conv.p, conv, conv_n * sizeof(T)));
     TRY(stage_upload_1d(C, C->in_uu.p, uu, N * sizeof(T)));
     TRY(stage_upload_1d(C, C->in_vv.p, vv, N * sizeof(T)));
@@ -1215,8 +1256,8 @@ int degrid_dev(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     TRY((prepare_tables<T, T>(C, P, support, ov, csh, reinterpret_cast<const T2*>(conv))));
     const int half = (ov + 1) / 2, noff = half + 1;
     // One job; its norm/skipped totals go to library scratch (not reported).
-    TRY(ensure(C->in_norm, sizeof(double)));
-    TRY(ensure(C->in_skipped, sizeof(unsigned long long)));
+    TRY(ensure(C, C->in_norm, sizeof(double)));
+    TRY(ensure(C, C->in_skipped, sizeof(unsigned long long)));
     C->h_jobs.resize(sizeof(JobDev<T>));
     JobDev<T>* hj = reinterpret_cast<JobDev<T>*>(C->h_jobs.data());
     hj->grid = const_cast<T2*>(reinterpret_cast<const T2*>(grid));
@@ -1237,21 +1278,22 @@ int degrid_dev(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     const unsigned long long chunks = (N + kSelChunk - 1) / kSelChunk;
     C->h_job_chunk_off.assign(2, 0);
     C->h_job_chunk_off[1] = (unsigned)chunks;
-    TRY(ensure(C->jobs_d, sizeof(JobDev<T>)));
-    TRY(ensure(C->job_chunk_off_d, 2 * sizeof(unsigned)));
+    TRY(ensure(C, C->jobs_d, sizeof(JobDev<T>)));
+    TRY(ensure(C, C->job_chunk_off_d, 2 * sizeof(unsigned)));
     HIP_TRY(hipMemcpyAsync(C->jobs_d.p, hj, sizeof(JobDev<T>), hipMemcpyHostToDevice, C->stream));
     HIP_TRY(hipMemcpyAsync(C->job_chunk_off_d.p, C->h_job_chunk_off.data(), 2 * sizeof(unsigned), hipMemcpyHostToDevice, C->stream));
-    TRY(ensure(C->meta, N * sizeof(int4)));
-    TRY(ensure(C->amp, N * sizeof(T4)));
-    TRY(ensure(C->num_visits, N * sizeof(unsigned)));
-    TRY(ensure(C->part_norm, chunks * sizeof(double)));
-    TRY(ensure(C->part_skipped, chunks * sizeof(unsigned long long)));
-    TRY(ensure(C->part_taps, chunks * sizeof(unsigned long long)));
-    TRY(ensure(C->part_nonfinite, chunks * sizeof(unsigned long long)));
-    TRY(ensure(C->part_bbox, chunks * sizeof(int4)));
-    SelectArgs<T, T> sa;
+    TRY(ensure(C, C->meta, N * sizeof(int4)));
+    TRY(ensure(C, C->amp, N * sizeof(T4)));
+    TRY(ensure(C, C->num_visits, N * sizeof(unsigned)));
+    TRY(ensure(C, C->part_norm, chunks * sizeof(double)));
+    TRY(ensure(C, C->part_skipped, chunks * sizeof(unsigned long long)));
+    TRY(ensure(C, C->part_taps, chunks * sizeof(unsigned long long)));
+    TRY(ensure(C, C->part_nonfinite, chunks * sizeof(unsigned long long)));
+    TRY(ensure(C, C->part_bbox, chunks * sizeof(int4)));
+    SelectArgs<T, T> sa{};
     sa.uu = uu; sa.vv = vv; sa.ww = ww; sa.weight = weight;
     sa.vis = nullptr;      // the adjoint needs the selection only, no amplitudes
+    sa.tw = kTileW; sa.th = kTileH;   // (tile-visit counts are computed, not used)
     sa.jobs = C->jobs_d.as<JobDev<T>>(); sa.num_jobs = 1;
     sa.job_chunk_off = C->job_chunk_off_d.as<unsigned>();
     sa.num_planes = (int)P; sa.support = C->support_d.as<int>();
@@ -1303,6 +1345,28 @@ int batched_jobs(size_t P, const int* support, int ov, int csh, const float* con
     }
     if (n && (!uu || !vv || !ww || !vis || !weight))
         return fail(MI355_ERR_INVALID_ARG, "NULL visibility array");
+    // Each job's outputs are written by plain read-modify-writes from many
+    // workgroups at once, so no two jobs may share a grid cell, a norm or a
+    // count: sort the output ranges by address and check neighbours.
+    struct Range { uintptr_t lo, hi; int job; };
+    for (int kind = 0; kind < 3; ++kind)
+    {
+        std::vector<Range> r((size_t)num_jobs);
+        for (int j = 0; j < num_jobs; ++j)
+        {
+            const uintptr_t p = kind == 0 ? (uintptr_t)jobs[j].grid
+                    : kind == 1 ? (uintptr_t)jobs[j].norm : (uintptr_t)jobs[j].num_skipped;
+            const size_t bytes = kind == 0 ? (size_t)std::max(jobs[j].grid_size, 0) *
+                    (size_t)std::max(jobs[j].grid_size, 0) * sizeof(float2)
+                    : kind == 1 ? sizeof(double) : sizeof(unsigned long long);
+            r[j] = {p, p + std::max<size_t>(bytes, 1), j};
+        }
+        std::sort(r.begin(), r.end(), [](const Range& a, const Range& b) { return a.lo < b.lo; });
+        for (size_t k = 1; k < r.size(); ++k)
+            if (r[k].lo < r[k - 1].hi)
+                return fail(MI355_ERR_INVALID_ARG, "jobs %d and %d share their %s", r[k - 1].job,
+                        r[k].job, kind == 0 ? "grid" : kind == 1 ? "norm" : "num_skipped");
+    }
     return MI355_OK;
 }
 
@@ -1327,8 +1391,8 @@ int weights_sorted_cells(mi355_ctx* C, size_t N, const T* uu, const T* vv, T gs,
 {
     for (int b = 0; b < 2; ++b)
     {
-        TRY(ensure(C->keys[b], N * sizeof(unsigned)));
-        TRY(ensure(C->vals[b], N * sizeof(unsigned)));
+        TRY(ensure(C, C->keys[b], N * sizeof(unsigned)));
+        TRY(ensure(C, C->vals[b], N * sizeof(unsigned)));
     }
     const unsigned long long cells = (unsigned long long)G * G;
     hipLaunchKernelGGL(k_weight_cells<T>, dim3(grid_blocks(N, 256, 16384)), dim3(256), 0,
@@ -1347,7 +1411,7 @@ int weights_write_dev(mi355_ctx* C, size_t N, const T* uu, const T* vv, const T*
     if (!skipped || !grid) return fail(MI355_ERR_INVALID_ARG, "NULL output pointer");
     if (N && (!uu || !vv || !weight)) return fail(MI355_ERR_INVALID_ARG, "NULL input array");
     HIP_TRY(hipSetDevice(C->device));
-    TRY(ensure(C->scalars, SC_COUNT * sizeof(unsigned long long)));
+    TRY(ensure(C, C->scalars, SC_COUNT * sizeof(unsigned long long)));
     unsigned long long* sc = C->scalars.as<unsigned long long>();
     hipLaunchKernelGGL(k_weight_init, dim3(1), dim3(64), 0, C->stream, sc + SC_W_COUNT,
             (unsigned long long)N, skipped, (int*)nullptr);
@@ -1392,7 +1456,7 @@ template <typename T>
 int upload_weight_window(mi355_ctx* C, const T* grid, int G, const int* box)
 {
     const size_t ww = (size_t)(box[2] - box[0] + 1), wh = (size_t)(box[3] - box[1] + 1);
-    TRY(ensure(C->in_grid, ww * wh * sizeof(T)));
+    TRY(ensure(C, C->in_grid, ww * wh * sizeof(T)));
     return stage_upload(C, C->in_grid.p, ww * sizeof(T), grid + (size_t)box[1] * G + box[0],
             (size_t)G * sizeof(T), ww * sizeof(T), wh);
 }
@@ -1409,11 +1473,11 @@ int weights_write_host(size_t N, const T* uu, const T* vv, const T* weight, T ce
     mi355_ctx* C = nullptr;
     TRY(get_default_ctx(&C));
     HIP_TRY(hipSetDevice(C->device));
-    TRY(ensure(C->in_uu, N * sizeof(T)));
-    TRY(ensure(C->in_vv, N * sizeof(T)));
-    TRY(ensure(C->in_weight, N * sizeof(T)));
-    TRY(ensure(C->in_skipped, sizeof(unsigned long long)));
-    TRY(ensure(C->scalars, SC_COUNT * sizeof(unsigned long long)));
+    TRY(ensure(C, C->in_uu, N * sizeof(T)));
+    TRY(ensure(C, C->in_vv, N * sizeof(T)));
+    TRY(ensure(C, C->in_weight, N * sizeof(T)));
+    TRY(ensure(C, C->in_skipped, sizeof(unsigned long long)));
+    TRY(ensure(C, C->scalars, SC_COUNT * sizeof(unsigned long long)));
     TRY(stage_upload_1d(C, C->in_uu.p, uu, N * sizeof(T)));
     TRY(stage_upload_1d(C, C->in_vv.p, vv, N * sizeof(T)));
     TRY(stage_upload_1d(C, C->in_weight.p, weight, N * sizeof(T)));
@@ -1461,12 +1525,12 @@ int weights_read_host(size_t N, const T* uu, const T* vv, const T* weight_in, T*
     mi355_ctx* C = nullptr;
     TRY(get_default_ctx(&C));
     HIP_TRY(hipSetDevice(C->device));
-    TRY(ensure(C->in_uu, N * sizeof(T)));
-    TRY(ensure(C->in_vv, N * sizeof(T)));
-    TRY(ensure(C->in_weight, N * sizeof(T)));
-    TRY(ensure(C->in_weight_out, N * sizeof(T)));
-    TRY(ensure(C->in_skipped, sizeof(unsigned long long)));
-    TRY(ensure(C->scalars, SC_COUNT * sizeof(unsigned long long)));
+    TRY(ensure(C, C->in_uu, N * sizeof(T)));
+    TRY(ensure(C, C->in_vv, N * sizeof(T)));
+    TRY(ensure(C, C->in_weight, N * sizeof(T)));
+    TRY(ensure(C, C->in_weight_out, N * sizeof(T)));
+    TRY(ensure(C, C->in_skipped, sizeof(unsigned long long)));
+    TRY(ensure(C, C->scalars, SC_COUNT * sizeof(unsigned long long)));
     TRY(stage_upload_1d(C, C->in_uu.p, uu, N * sizeof(T)));
     TRY(stage_upload_1d(C, C->in_vv.p, vv, N * sizeof(T)));
     unsigned long long* sc = C->scalars.as<unsigned long long>();
@@ -1528,7 +1592,7 @@ int image_dev(mi355_ctx* C, int G, const T* grid, double norm, T* image)
     HIP_TRY(hipSetDevice(C->device));
     typedef typename V2<T>::t T2;
     const unsigned long long n = (unsigned long long)G * G;
-    TRY(ensure(C->fft_work, n * sizeof(T2)));
+    TRY(ensure(C, C->fft_work, n * sizeof(T2)));
     const int dbl = sizeof(T) == 8;
     if (!C->fft_valid || C->fft_G != G || C->fft_double != dbl)
     {
@@ -1567,8 +1631,8 @@ int image_host(int G, const T* grid, double norm, T* image)
     TRY(get_default_ctx(&C));
     HIP_TRY(hipSetDevice(C->device));
     const size_t n = (size_t)G * G;
-    TRY(ensure(C->in_grid, 2 * n * sizeof(T)));
-    TRY(ensure(C->in_image, n * sizeof(T)));
+    TRY(ensure(C, C->in_grid, 2 * n * sizeof(T)));
+    TRY(ensure(C, C->in_image, n * sizeof(T)));
     TRY(stage_upload_1d(C, C->in_grid.p, grid, 2 * n * sizeof(T)));
     TRY(image_dev<T>(C, G, C->in_grid.as<T>(), norm, C->in_image.as<T>()));
     TRY(stage_download_1d(C, image, C->in_image.p, n * sizeof(T)));
@@ -1639,12 +1703,14 @@ MI355_EXPORT int mi355_ctx_create(int device, mi355_ctx** ctx)
 
 MI355_EXPORT int mi355_ctx_destroy(mi355_ctx* C)
 {
-    if (C && C->graph_exec) (void)hipGraphExecDestroy(C->graph_exec);
-    if (C && C->graph) (void)hipGraphDestroy(C->graph);
-    if (C && C->capture_stream) (void)hipStreamDestroy(C->capture_stream);
     if (!C) return ok();
+    if (C->graph_ctx) (void)mi355_ctx_destroy(C->graph_ctx);
+    C->graph_ctx = nullptr;
+    if (C->graph_exec) (void)hipGraphExecDestroy(C->graph_exec);
+    if (C->graph) (void)hipGraphDestroy(C->graph);
+    if (C->capture_stream) (void)hipStreamDestroy(C->capture_stream);
     (void)hipSetDevice(C->device);
-    (void)hipStreamSynchronize(C->stream);
+    (void)wait_ctx_work(C);
     DevBuf* all[] = {&C->tables, &C->norm_table, &C->support_d, &C->plane_off_d, &C->meta,
         &C->amp, &C->num_visits, &C->visit_off, &C->part_norm, &C->part_skipped,
         &C->part_taps, &C->part_nonfinite, &C->part_bbox, &C->scan_partial, &C->scalars,
@@ -1657,13 +1723,14 @@ MI355_EXPORT int mi355_ctx_destroy(mi355_ctx* C)
         &C->in_vis, &C->in_weight, &C->in_grid, &C->in_norm, &C->in_skipped, &C->in_weight_out,
         &C->fft_work, &C->in_image, &C->vmeta, &C->vamp, &C->vlds};
     if (C->fft_valid) (void)hipfftDestroy(C->fft_plan);
-    for (DevBuf* b : all) release(*b);
+    for (DevBuf* b : all) release(C, *b);
     for (int i = 0; i < 2; ++i)
     {
         release_host(C->h_stage[i]);
         if (C->stage_ev[i]) (void)hipEventDestroy(C->stage_ev[i]);
     }
     for (int i = 0; i < EV_COUNT; ++i) if (C->ev[i]) (void)hipEventDestroy(C->ev[i]);
+    if (C->switch_ev) (void)hipEventDestroy(C->switch_ev);
     delete C;
     return ok();
 }
@@ -1671,7 +1738,17 @@ MI355_EXPORT int mi355_ctx_destroy(mi355_ctx* C)
 MI355_EXPORT int mi355_ctx_set_stream(mi355_ctx* C, void* stream)
 {
     if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
-    C->stream = reinterpret_cast<hipStream_t>(stream);
+    const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (s != C->stream)
+    {
+        // mark the end of the work queued so far: a buffer that grows later
+        // is freed only once that work is done
+        HIP_TRY(hipSetDevice(C->device));
+        if (C->switch_pending) HIP_TRY(hipEventSynchronize(C->switch_ev));
+        HIP_TRY(hipEventRecord(C->switch_ev, C->stream));
+        C->switch_pending = true;
+    }
+    C->stream = s;
     return ok();
 }
 
@@ -1699,6 +1776,10 @@ MI355_EXPORT int mi355_ctx_set_option(mi355_ctx* C, int key, long long value)
         if (value < 1 || value > 4096) return fail(MI355_ERR_INVALID_ARG, "item group %lld", value);
         C->item_group = (unsigned)value;
         break;
+    case MI355_OPT_MAX_WORKSPACE:
+        if (value < 0) return fail(MI355_ERR_INVALID_ARG, "workspace cap %lld", value);
+        C->max_workspace = (unsigned long long)value;
+        break;
     default: return fail(MI355_ERR_INVALID_ARG, "unknown option %d", key);
     }
     return ok();
@@ -1716,6 +1797,7 @@ MI355_EXPORT int mi355_ctx_get_option(const mi355_ctx* C, int key, long long* va
     case MI355_OPT_GRID_KERNEL: *value = C->grid_kernel; break;
     case MI355_OPT_MIN_STAGE_RUN: *value = C->min_stage_run; break;
     case MI355_OPT_ITEM_GROUP: *value = C->item_group; break;
+    case MI355_OPT_MAX_WORKSPACE: *value = (long long)C->max_workspace; break;
     default: return fail(MI355_ERR_INVALID_ARG, "unknown option %d", key);
     }
     return ok();
@@ -1725,6 +1807,7 @@ MI355_EXPORT int mi355_ctx_get_stats(const mi355_ctx* Cc, mi355_stats* stats)
 {
     if (!Cc || !stats) return fail(MI355_ERR_INVALID_ARG, "NULL argument");
     mi355_ctx* C = const_cast<mi355_ctx*>(Cc);
+    if (C->stats_in_graph && C->graph_ctx) C = C->graph_ctx;
     if (C->stats_deferred)
     {
         // host-sync-free call: read the counts the device kept
@@ -1850,6 +1933,28 @@ MI355_EXPORT int mi355_grid_wproj_f_batched_graph_dev(mi355_ctx* C, size_t P,
     std::vector<JobHost<float, float>> J;
     TRY(batched_jobs(P, support, ov, csh, conv, num_jobs, jobs, uu, vv, ww, vis, weight, J));
     HIP_TRY(hipSetDevice(C->device));
+    // The graph's own context: its buffers, device job table, support and
+    // plane offsets are written only by the graph's eager build run, so
+    // calls of any other entry point on C leave a captured graph intact.
+    if (!C->graph_ctx)
+    {
+        mi355_ctx* Gc = new mi355_ctx();
+        const int s = init_ctx(Gc, C->device);
+        if (s != MI355_OK) { delete Gc; return s; }
+        C->graph_ctx = Gc;
+    }
+    mi355_ctx* const parent = C;
+    C = parent->graph_ctx;
+    C->visits_per_item = parent->visits_per_item;
+    C->compensated = parent->compensated;
+    C->grid_kernel = parent->grid_kernel;
+    C->min_stage_run = parent->min_stage_run;
+    C->item_group = parent->item_group;
+    C->max_workspace = parent->max_workspace;
+    C->timing = parent->timing;
+    C->cache_kernels = false;
+    if (C->stream != parent->stream) TRY(mi355_ctx_set_stream(C, parent->stream));
+    parent->stats_in_graph = true;
     // What the captured launch depends on: shapes, pointers, job table,
     // options.  The input values themselves may change between launches.
     std::vector<unsigned char> key;
@@ -1858,8 +1963,9 @@ MI355_EXPORT int mi355_grid_wproj_f_batched_graph_dev(mi355_ctx* C, size_t P,
         key.insert(key.end(), b, b + n);
     };
     const void* ptrs[6] = {conv, uu, vv, ww, vis, weight};
-    const long long opts[5] = {(long long)C->visits_per_item, (long long)C->min_stage_run,
-                               C->grid_kernel, C->compensated, (long long)(intptr_t)C->stream};
+    const long long opts[7] = {(long long)C->visits_per_item, (long long)C->min_stage_run,
+                               C->grid_kernel, C->compensated, (long long)C->item_group,
+                               (long long)C->max_workspace, (long long)(intptr_t)C->stream};
     put(&P, sizeof(P));
     put(support, P * sizeof(int));
     put(&ov, sizeof(ov));
@@ -1953,9 +2059,9 @@ MI355_EXPORT int mi355_autotune_f_batched_dev(mi355_ctx* C, size_t P, const int*
     int gmax = 0;
     for (int j = 0; j < num_jobs; ++j) gmax = std::max(gmax, J[j].grid_size);
     DevBuf tgrid, tnorm, tskip;
-    TRY(ensure(tgrid, (size_t)gmax * gmax * sizeof(float2)));
-    TRY(ensure(tnorm, (size_t)num_jobs * sizeof(double)));
-    TRY(ensure(tskip, (size_t)num_jobs * sizeof(unsigned long long)));
+    TRY(ensure(C, tgrid, (size_t)gmax * gmax * sizeof(float2)));
+    TRY(ensure(C, tnorm, (size_t)num_jobs * sizeof(double)));
+    TRY(ensure(C, tskip, (size_t)num_jobs * sizeof(unsigned long long)));
     for (int j = 0; j < num_jobs; ++j)
     {
         J[j].grid = tgrid.as<float2>();         // (results are discarded)
@@ -1998,7 +2104,7 @@ MI355_EXPORT int mi355_autotune_f_batched_dev(mi355_ctx* C, size_t P, const int*
     C->visits_per_item = st == MI355_OK ? bc.vpi : vpi0;
     C->min_stage_run = st == MI355_OK ? bc.msr : msr0;
     C->item_group = st == MI355_OK ? bc.group : ig0;
-    release(tgrid); release(tnorm); release(tskip);
+    release(C, tgrid); release(C, tnorm); release(C, tskip);
     if (st != MI355_OK) return st;
     if (best_ms) *best_ms = best;
     return ok();

@@ -820,6 +820,11 @@ MI355_EXPORT int mi355_fits_read_kernel(const char* path, int precision, mi355_k
             if (!bp || rep < 1 || rep * rows < num)
             { st = fail(MI355_ERR_FORMAT, "%s: unable to read kernel support sizes (TFORM1 '%s')",
                     path, tform.c_str()); break; }
+            // the column's elements must lie inside a row (element i is read
+            // at row i / rep, byte (i % rep) * width)
+            if (row < 1 || rows < 1 || rep > row / (std::abs(bp) / 8) || row > (1ll << 40) / rows)
+            { st = fail(MI355_ERR_FORMAT, "%s: W_KERNELS column (TFORM1 '%s') does not fit a "
+                    "row of %lld bytes", path, tform.c_str(), row); break; }
             double tscal = 1.0, tzero = 0.0;
             if (t->keys.count("TSCAL1")) parse_double(t->keys.at("TSCAL1"), &tscal);
             if (t->keys.count("TZERO1")) parse_double(t->keys.at("TZERO1"), &tzero);

@@ -34,6 +34,7 @@ OPT_CACHE_KERNELS = 4
 OPT_GRID_KERNEL = 5
 OPT_MIN_STAGE_RUN = 6
 OPT_ITEM_GROUP = 7
+OPT_MAX_WORKSPACE = 8
 
 
 class GridderError(RuntimeError):

@@ -89,11 +89,13 @@ def make(config, num_points=None, grid_size=None, num_w_planes=None, max_support
 
 
 def make_batched(num_jobs=64, grid_size=4096, num_w_planes=339, max_support=44,
-                 n_min=50_000, n_max=2_000_000, seed=SEED_BASE + 3):
+                 n_min=50_000, n_max=2_000_000, seed=SEED_BASE + 3, counts=None):
     """The batched workload: num_jobs independent grids sharing one kernel stack, with
-    visibility counts log-uniform in [n_min, n_max]."""
+    visibility counts log-uniform in [n_min, n_max] (or the given counts)."""
     rng = np.random.default_rng(seed)
-    counts = np.exp(rng.uniform(np.log(n_min), np.log(n_max), num_jobs)).astype(np.int64)
+    counts = np.exp(rng.uniform(np.log(n_min), np.log(n_max), num_jobs)).astype(np.int64) \
+        if counts is None else np.asarray(counts, np.int64)
+    num_jobs = len(counts)
     support, conv, csh = kernels(num_w_planes, max_support, 4)
     cell = cell_size(grid_size, max_support)
     jobs = []

@@ -107,6 +107,10 @@ MI355_EXPORT int mi355_ctx_set_stream(mi355_ctx* ctx, void* hip_stream);
 #define MI355_OPT_ITEM_GROUP      7  /* strip kernel: work items are dealt to the
                                         8 XCDs in groups of this many consecutive
                                         items (default 1: round-robin) */
+#define MI355_OPT_MAX_WORKSPACE   8  /* cap in bytes on the device workspace the
+                                        context may hold (0: no cap, the default);
+                                        a call that would need more fails with
+                                        MI355_ERR_OUT_OF_MEMORY before allocating */
 MI355_EXPORT int mi355_ctx_set_option(mi355_ctx* ctx, int key, long long value);
 MI355_EXPORT int mi355_ctx_get_option(const mi355_ctx* ctx, int key, long long* value);
 
@@ -207,6 +211,10 @@ MI355_EXPORT int mi355_degrid_wproj_d_dev(mi355_ctx* ctx,
  * visibilities [vis_offset, vis_offset + num_points) of the concatenated
  * arrays onto its own grid.  The reference grids one block per call
  * (main.cpp:243-381); this is the same operation for many blocks at once.
+ * Every job needs outputs of its own: grids, norms and counts that overlap
+ * another job's are refused (MI355_ERR_INVALID_ARG), since the jobs'
+ * outputs are written concurrently.  (main.cpp's blocks, which all add to
+ * one grid, are gridded by calling a single-job entry point per block.)
  * ---------------------------------------------------------------------- */
 typedef struct
 {

@@ -29,6 +29,8 @@
  *   oracle_grid_wproj_fsel_d_tiles  the same FP64 truth restricted to a list
  *                              of 64x64 grid tiles, multi-threaded, for
  *                              spot checks of full-size grids.
+ *   oracle_norm_f              norm and num_skipped of oracle_grid_wproj_f
+ *                              without the grid (bit-identical to it).
  *
  * Uniform weighting (the pre-pass upstream of the gridder), restated from
  * the reference's serial functions, same precision and order:
@@ -205,6 +207,55 @@ ORACLE_API void oracle_grid_wproj_f(size_t num_w_planes, const int* support,
             *norm += tap_sum * w_i;
         }
     }
+}
+
+/* Norm and num_skipped of oracle_grid_wproj_f without the grid: the same
+ * per-visibility double sum of Re(tap) over the footprint, rows j outer and
+ * columns k inner (oskar_grid_wproj.c:190-207), added as sum * weight in
+ * visibility order (:210).  The sum depends only on (plane, off_u, off_v),
+ * so each distinct one is computed once, by the same loop, and reused: the
+ * result is bit-identical to oracle_grid_wproj_f's norm.  For checking the
+ * norm of workloads too large for the full serial gridder. */
+ORACLE_API void oracle_norm_f(size_t num_w_planes, const int* support,
+        int oversample, int conv_size_half, const float* conv_func,
+        size_t num_points, const float* uu, const float* vv, const float* ww,
+        const float* weight, float cell_size_rad, float w_scale, int grid_size,
+        size_t* num_skipped, double* norm)
+{
+    const size_t plane_len = (size_t)conv_size_half * conv_size_half;
+    const float grid_scale = grid_size * cell_size_rad;
+    const int side = 2 * oversample + 1;          /* offsets in [-ov, ov] */
+    double* memo = (double*)malloc(num_w_planes * side * side * sizeof(double));
+    unsigned char* have = (unsigned char*)calloc(num_w_planes * side * side, 1);
+    size_t i;
+    *num_skipped = 0;
+    for (i = 0; i < num_points; ++i)
+    {
+        oracle_sel s;
+        size_t e;
+        select_f(uu[i], vv[i], ww[i], grid_scale, w_scale, grid_size,
+                oversample, num_w_planes, support, &s);
+        if (s.skip) { *num_skipped += 1; continue; }
+        e = (s.plane * side + (size_t)(s.off_v + oversample)) * side + (size_t)(s.off_u + oversample);
+        if (!have[e])
+        {
+            const float* plane = conv_func + 2 * s.plane * plane_len;
+            double tap_sum = 0.0;
+            int j, k;
+            for (j = -s.support; j <= s.support; ++j)
+            {
+                const float* krow = plane +
+                        2 * (size_t)abs(s.off_v + j * oversample) * conv_size_half;
+                for (k = -s.support; k <= s.support; ++k)
+                    tap_sum += krow[2 * abs(s.off_u + k * oversample)];
+            }
+            memo[e] = tap_sum;
+            have[e] = 1;
+        }
+        *norm += memo[e] * weight[i];
+    }
+    free(memo);
+    free(have);
 }
 
 /* FP64 gridder, oskar_grid_wproj_d (oskar_grid_wproj.h:68-86,

@@ -58,9 +58,10 @@ def oracle():
                                                    P, I, P]
         lib.oracle_grid_wproj_fsel_d_tiles.argtypes = [SZ, P, I, I, P, SZ, P, P, P, P, P, F, F,
                                                        I, I, P, SZ, I, P]
+        lib.oracle_norm_f.argtypes = [SZ, P, I, I, P, SZ, P, P, P, P, F, F, I, P, P]
         for fn in (lib.oracle_grid_wproj_f, lib.oracle_grid_wproj_d, lib.oracle_grid_wproj_fsel_d,
                    lib.oracle_select_f, lib.oracle_select_d, lib.oracle_degrid_wproj_fsel_d,
-                   lib.oracle_grid_wproj_fsel_d_tiles):
+                   lib.oracle_grid_wproj_fsel_d_tiles, lib.oracle_norm_f):
             fn.restype = None
         _weights_sigs(lib, "oracle_grid_weights_")
         lib.oracle_grid_to_image_d.argtypes = [I, P, D, P]
@@ -147,6 +148,22 @@ def grid_fsel_d(case, grid=None, norm=0.0):
         ctypes.c_float(float(case["cell_size_rad"])), ctypes.c_float(float(case["w_scale"])),
         G, ctypes.byref(skipped), ctypes.byref(nrm), _p(grid))
     return grid, nrm.value, skipped.value
+
+
+def norm_f(case, norm=0.0):
+    """(norm, num_skipped) of oskar_grid_wproj_f on case, without the grid
+    (bit-identical to grid_f's)."""
+    lib = oracle()
+    x = _inputs(case, np.float32)
+    skipped = ctypes.c_size_t(0)
+    nrm = ctypes.c_double(norm)
+    lib.oracle_norm_f(int(case["num_w_planes"]), _p(x["support"]), int(case["oversample"]),
+                      int(case["conv_size_half"]), _p(x["conv"]), int(case["num_points"]),
+                      _p(x["uu"]), _p(x["vv"]), _p(x["ww"]), _p(x["weight"]),
+                      ctypes.c_float(float(case["cell_size_rad"])),
+                      ctypes.c_float(float(case["w_scale"])), int(case["grid_size"]),
+                      ctypes.byref(skipped), ctypes.byref(nrm))
+    return nrm.value, skipped.value
 
 
 def select_f(case):

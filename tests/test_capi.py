@@ -195,3 +195,47 @@ def test_weights_zero_points_and_nulls_without_device():
     lib.mi355_grid_weights_read_f(4, None, None, None, None, ctypes.c_float(0.1), 8,
                                   ctypes.byref(sk), ctypes.c_void_p(g.ctypes.data))
     assert lib.mi355_grid_last_error() == _lib.ERR_INVALID_ARG
+
+
+def _jobs(grids, norms, skips, G=64, n=10):
+    jobs = (_lib.JobF * len(grids))()
+    for j, (g, nr, sk) in enumerate(zip(grids, norms, skips)):
+        jobs[j].vis_offset = j * n
+        jobs[j].num_points = n
+        jobs[j].cell_size_rad = 0.01
+        jobs[j].w_scale = 1.0
+        jobs[j].grid_size = G
+        jobs[j].grid = g
+        jobs[j].norm = nr
+        jobs[j].num_skipped = sk
+    return jobs
+
+
+@pytest.mark.parametrize("case", ["same_grid", "overlapping_grid", "same_norm", "same_skipped"])
+def test_batched_jobs_with_shared_outputs_rejected(case):
+    """Jobs' outputs are written concurrently, so a job table whose jobs share
+    a grid cell, a norm or a count is refused before the device is touched
+    (the addresses below are never dereferenced)."""
+    lib = _lib.load()
+    G = 64
+    gbytes = G * G * 8
+    grids = [0x10000000, 0x20000000, 0x30000000]
+    norms = [0x40000000, 0x40000008, 0x40000010]
+    skips = [0x50000000, 0x50000008, 0x50000010]
+    if case == "same_grid":
+        grids[2] = grids[0]
+    elif case == "overlapping_grid":
+        grids[1] = grids[0] + gbytes - 8          # last cell of job 0 = first of job 1
+    elif case == "same_norm":
+        norms[1] = norms[0]
+    else:
+        skips[2] = skips[1]
+    support, conv, csh = synth.kernels(2, 2, 4)
+    arr = np.zeros(64, np.float32)
+    p = ctypes.c_void_p(arr.ctypes.data)
+    st = lib.mi355_grid_wproj_f_batched_dev(ctypes.c_void_p(1), 2,
+                                            ctypes.c_void_p(support.ctypes.data), 4, csh,
+                                            ctypes.c_void_p(conv.ctypes.data), 3,
+                                            _jobs(grids, norms, skips, G), p, p, p, p, p)
+    assert st == _lib.ERR_INVALID_ARG
+    assert b"share" in lib.mi355_grid_last_error_message()

@@ -134,3 +134,79 @@ def test_autotune_keeps_results_and_sets_options(gpu):
     torch.cuda.synchronize()
     for job in jobs:
         assert job["num_skipped"].item() >= 0
+
+
+def test_graph_survives_other_calls_on_the_context(gpu):
+    """A captured batched graph is private to its context's graph state: single
+    grid / degrid / selection calls and the autotuner on the same context in
+    between (which rewrite the context's own job table, support and plane
+    offsets, point jobs at other grids and grow or free buffers) leave later
+    replays equal to the eager pass bit for bit."""
+    torch, ops = gpu
+    ks, cat, jobs = _batched_case(torch, ops, num_jobs=4, seed=6)
+    args = (ks, jobs, cat["uu"], cat["vv"], cat["ww"], cat["vis"], cat["weight"])
+    ctx = ops.context()
+    _zero(jobs)
+    ops.grid_batched(*args, ctx=ctx)
+    torch.cuda.synchronize()
+    ref = _snap(jobs)
+
+    def check():
+        _zero(jobs)
+        ops.grid_batched_graph(*args, ctx=ctx)
+        torch.cuda.synchronize()
+        for (g0, n0, s0), (g1, n1, s1) in zip(ref, _snap(jobs)):
+            assert same_bits(g0, g1) and n0 == n1 and s0 == s1
+
+    check()                            # build
+    check()                            # replay
+    # other entry points on the same context, with a different (larger) stack
+    b = synth.make_batched(num_jobs=1, grid_size=2048, num_w_planes=96, max_support=28,
+                           n_min=60_000, n_max=60_001, seed=11)
+    j = b["jobs"][0]
+    ks2 = ops.KernelStack(b["support"], b["oversample"], b["conv_size_half"], b["conv_func"])
+    d = {k: torch.from_numpy(np.ascontiguousarray(j[k])).cuda()
+         for k in ("uu", "vv", "ww", "vis", "weight")}
+    G = j["grid_size"]
+    g = torch.zeros((G, G, 2), dtype=torch.float32, device="cuda")
+    n = torch.zeros(1, dtype=torch.float64, device="cuda")
+    s = torch.zeros(1, dtype=torch.int64, device="cuda")
+    ops.grid(ks2, d["uu"], d["vv"], d["ww"], d["vis"], d["weight"], j["cell_size_rad"],
+             j["w_scale"], G, g, n, s, ctx=ctx)
+    check()
+    ops.degrid(ks2, d["uu"], d["vv"], d["ww"], d["weight"], j["cell_size_rad"], j["w_scale"],
+               G, g, ctx=ctx)
+    ops.select(b["support"], b["oversample"], d["uu"], d["vv"], d["ww"], j["cell_size_rad"],
+               j["w_scale"], G, ctx=ctx)
+    check()
+    ops.autotune_batched(*args, ctx=ctx)
+    check()
+    assert ctx.stats()["num_points"] == cat["uu"].numel()
+
+
+def test_workspace_cap_refuses_before_allocating(gpu):
+    """MI355_OPT_MAX_WORKSPACE: a call whose buffers would exceed the cap
+    fails with MI355_ERR_OUT_OF_MEMORY (forced from the host, no device
+    allocation is attempted) and leaves the outputs alone; lifting the cap
+    lets the same call through."""
+    torch, ops = gpu
+    ks, cat, jobs = _batched_case(torch, ops, num_jobs=2, seed=7)
+    ctx = _lib.Context(torch.cuda.current_device())
+    try:
+        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        ctx.set_option(_lib.OPT_MAX_WORKSPACE, 1 << 20)
+        assert ctx.get_option(_lib.OPT_MAX_WORKSPACE) == 1 << 20
+        _zero(jobs)
+        with pytest.raises(_lib.GridderError) as e:
+            ops.grid_batched(ks, jobs, cat["uu"], cat["vv"], cat["ww"], cat["vis"],
+                             cat["weight"], ctx=ctx)
+        assert e.value.status == _lib.ERR_OUT_OF_MEMORY and "cap" in str(e.value)
+        torch.cuda.synchronize()
+        assert all(job["grid"].abs().max().item() == 0 for job in jobs)
+        ctx.set_option(_lib.OPT_MAX_WORKSPACE, 0)
+        ops.grid_batched(ks, jobs, cat["uu"], cat["vv"], cat["ww"], cat["vis"], cat["weight"],
+                         ctx=ctx)
+        torch.cuda.synchronize()
+        assert all(job["num_skipped"].item() >= 0 for job in jobs)
+    finally:
+        ctx.close()

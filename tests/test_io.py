@@ -346,3 +346,17 @@ def test_kernel_round_trip_and_errors(tmp_path):
     with pytest.raises(_lib.GridderError) as e:
         vis_io.read_kernel(path)
     assert "3D" in str(e.value)
+
+
+def test_kernel_support_column_wider_than_row_refused(tmp_path):
+    """A W_KERNELS table whose TFORM1 repeat count times the element width
+    exceeds the row width (NAXIS1) is refused before any element is read."""
+    data = np.arange(2 * 4 * 4, dtype=np.float32).reshape(2, 4, 4)
+    path = str(tmp_path / "k.fits")
+    vis_io.write_kernel(path, data, [1, 2], 4, 64, 0.01, 1.0)
+    raw = open(path, "rb").read()
+    assert raw.count(b"'1J") == 1
+    open(path, "wb").write(raw.replace(b"'1J", b"'3K"))   # 24-byte column in 4-byte rows
+    with pytest.raises(_lib.GridderError) as e:
+        vis_io.read_kernel(path)
+    assert e.value.status == _lib.ERR_FORMAT and "does not fit" in str(e.value)

@@ -263,3 +263,25 @@ def test_tile_oracle_matches_full_grid():
     t = oracle.tiles_fsel_d(c, tiles, tile=64, threads=2)
     for k, (tx, ty) in enumerate(tiles):
         assert np.array_equal(t[k], g[ty * 64:(ty + 1) * 64, tx * 64:(tx + 1) * 64])
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_norm_oracle_equals_gridder_norm(name):
+    """oracle_norm_f (norm without the grid) reproduces the compiled
+    reference's FP32 norm and num_skipped bit for bit on every fixture, and
+    oracle_grid_wproj_f's on fresh random cases (it memoises the same
+    per-footprint sum)."""
+    c = load_golden(name)
+    n0 = float(c["norm0"]) if "grid0" in c else 0.0
+    n, s = oracle.norm_f(c, norm=n0)
+    if "grid0" in c:
+        n, s = oracle.norm_f(c, norm=n)
+    assert same_scalar(n, c["norm_f"]) and s == int(c["skipped_f"])
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_norm_oracle_random(seed):
+    c = _random_case(seed, N=3000)
+    _, n, s = oracle.grid_f(c)
+    n2, s2 = oracle.norm_f(c)
+    assert n2 == n and s2 == s
