@@ -311,7 +311,8 @@ void k_select_out(const TI* __restrict__ uu, const TI* __restrict__ vv,
 
 // Per-job totals over the job's chunks, in chunk order.  job_out receives
 // {skipped, taps, non-finite} and job_bbox the {x0, y0, x1, y1} cell box
-// (x1 < x0 if nothing is gridded).  *norm += sum; *skipped = count.
+// (x1 < x0 if nothing is gridded), job_norm the norm term sum.  The
+// caller's outputs are written by k_job_commit at the end of the pass.
 template <typename T>
 __global__ __launch_bounds__(kSelThreads)
 void k_job_reduce(const JobDev<T>* __restrict__ jobs,
@@ -321,7 +322,8 @@ void k_job_reduce(const JobDev<T>* __restrict__ jobs,
         const unsigned long long* __restrict__ part_taps,
         const unsigned long long* __restrict__ part_nonfinite,
         const int4* __restrict__ part_bbox,
-        unsigned long long* __restrict__ job_out, int4* __restrict__ job_bbox)
+        unsigned long long* __restrict__ job_out, int4* __restrict__ job_bbox,
+        double* __restrict__ job_norm)
 {
     __shared__ double sh_d[kSelThreads];
     __shared__ unsigned long long sh_u[kSelThreads];
@@ -348,13 +350,28 @@ void k_job_reduce(const JobDev<T>* __restrict__ jobs,
     b2 = block_min(b2, sh_i); b3 = block_min(b3, sh_i);
     if (threadIdx.x == 0)
     {
-        *jobs[j].norm += nsum;
-        *jobs[j].skipped = nskip;
+        job_norm[j] = nsum;
         job_out[3 * j] = nskip;
         job_out[3 * j + 1] = ntaps;
         job_out[3 * j + 2] = nnf;
         job_bbox[j] = make_int4(b0, b1, b2 == INT_MAX ? INT_MIN : -b2,
                 b3 == INT_MAX ? INT_MIN : -b3);
+    }
+}
+
+// The pass's outputs besides the grids: *norm += the job's norm sum (the
+// `*norm += sum * weight` of oskar_grid_wproj.c:210, summed per job),
+// *num_skipped = its count (set per call, :151).  Launched after the last
+// kernel that can fail to be launched, so that a call refused on the way
+// (workspace) leaves the caller's outputs as they were.
+template <typename T>
+__global__ void k_job_commit(const JobDev<T>* __restrict__ jobs, int num_jobs,
+        const double* __restrict__ job_norm, const unsigned long long* __restrict__ job_out)
+{
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < num_jobs; j += gridDim.x * blockDim.x)
+    {
+        *jobs[j].norm += job_norm[j];
+        *jobs[j].skipped = job_out[3 * j];
     }
 }
 

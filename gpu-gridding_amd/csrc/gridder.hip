@@ -52,6 +52,7 @@ int ok()
 
 } // namespace ggs
 
+
 namespace {
 
 using ggs::fail;
@@ -187,6 +188,10 @@ struct mi355_ctx
     // context was a graph call when stats_in_graph is set.
     mi355_ctx* graph_ctx = nullptr;
     bool stats_in_graph = false;
+    // the last strip-kernel pass's sorted tile lists (mi355_ctx_get_buckets)
+    bool bk_valid = false;
+    int bk_cur = 0, bk_key_table_bits = 0;
+    unsigned long long bk_count = 0;
 
     // prepared kernel tables
     const void* cache_conv = nullptr;
@@ -207,7 +212,7 @@ struct mi355_ctx
     DevBuf run_start, run_z, run_tb;
     DevBuf vmeta, vamp, vlds;       // sorted visit records (strip kernel)
     DevBuf scratch;
-    DevBuf jobs_d, job_chunk_off_d, job_vis_off_d, job_tile_base_d, job_out_d, job_bbox_d;
+    DevBuf jobs_d, job_chunk_off_d, job_vis_off_d, job_tile_base_d, job_out_d, job_bbox_d, job_norm_d;
     // drop-in staging
     DevBuf in_conv, in_uu, in_vv, in_ww, in_vis, in_weight, in_grid, in_norm, in_skipped;
     DevBuf in_weight_out;
@@ -332,6 +337,22 @@ int init_ctx(mi355_ctx* C, int device)
     for (int i = 0; i < EV_COUNT; ++i) HIP_TRY(hipEventCreate(&C->ev[i]));
     HIP_TRY(hipEventCreateWithFlags(&C->switch_ev, hipEventDisableTiming));
     return MI355_OK;
+}
+
+// Every device buffer a context holds.
+std::vector<DevBuf*> device_buffers(mi355_ctx* C)
+{
+    return {&C->tables, &C->norm_table, &C->support_d, &C->plane_off_d, &C->meta,
+        &C->amp, &C->num_visits, &C->visit_off, &C->part_norm, &C->part_skipped,
+        &C->part_taps, &C->part_nonfinite, &C->part_bbox, &C->scan_partial, &C->scalars,
+        &C->keys[0], &C->keys[1],
+        &C->vals[0], &C->vals[1], &C->hist, &C->hist_off, &C->tile_start, &C->tile_end,
+        &C->n_items, &C->n_slots, &C->item_off, &C->slot_off, &C->item_tile, &C->run_start,
+        &C->run_z, &C->run_tb, &C->scratch,
+        &C->jobs_d, &C->job_chunk_off_d, &C->job_vis_off_d, &C->job_tile_base_d,
+        &C->job_out_d, &C->job_bbox_d, &C->job_norm_d, &C->in_conv, &C->in_uu, &C->in_vv, &C->in_ww,
+        &C->in_vis, &C->in_weight, &C->in_grid, &C->in_norm, &C->in_skipped, &C->in_weight_out,
+        &C->fft_work, &C->in_image, &C->vmeta, &C->vamp, &C->vlds};
 }
 
 int get_default_ctx(mi355_ctx** out)
@@ -503,6 +524,7 @@ enum Scalar { SC_VISITS, SC_OVERFLOW, SC_ITEMS, SC_SLOTS, SC_SCRATCH0, SC_RUNS =
 int radix_sort(mi355_ctx* C, unsigned long long n_bound, const unsigned long long* d_n,
         int key_bits, int* cur_out)
 {
+    C->bk_valid = false;              // (the ping-pong buffers are rewritten)
     unsigned long long* sc = C->scalars.as<unsigned long long>();
     const int bits = (key_bits + 8) / 9 < (key_bits + 7) / 8 ? 9 : 8;
     const unsigned radix = 1u << bits;
@@ -571,6 +593,18 @@ int grid_nonfinite(mi355_ctx* C, unsigned long long count, int num_jobs,
     return MI355_OK;
 }
 
+// The outputs besides the grids (norm, num_skipped), after every other
+// kernel of the pass has been launched.
+template <typename T>
+int commit_jobs(mi355_ctx* C, int num_jobs)
+{
+    hipLaunchKernelGGL(k_job_commit<T>, dim3(grid_blocks(num_jobs, 64, 1024)), dim3(64), 0,
+            C->stream, C->jobs_d.as<JobDev<T>>(), num_jobs, C->job_norm_d.as<double>(),
+            C->job_out_d.as<unsigned long long>());
+    HIP_TRY(hipGetLastError());
+    return MI355_OK;
+}
+
 // Called once the gridded bounding box of each job is known (before the
 // grid is touched); the host drop-in uploads the caller's grid window here.
 struct BeforeGrid
@@ -590,6 +624,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     typedef typename V4<T>::t T4;
     C->stats = mi355_stats();
     C->stats_in_graph = false;
+    C->bk_valid = false;
     record(C, EV_START);
     TRY((prepare_tables<TI, TA>(C, P, support, ov, csh, conv)));
     record(C, EV_PREP);
@@ -673,6 +708,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     TRY(ensure(C, C->job_vis_off_d, (num_jobs + 1) * sizeof(unsigned long long)));
     TRY(ensure(C, C->job_out_d, 3 * num_jobs * sizeof(unsigned long long)));
     TRY(ensure(C, C->job_bbox_d, num_jobs * sizeof(int4)));
+    TRY(ensure(C, C->job_norm_d, num_jobs * sizeof(double)));
     if (!C->capturing)
     {
         HIP_TRY(hipMemcpyAsync(C->jobs_d.p, C->h_jobs.data(), C->h_jobs.size(), hipMemcpyHostToDevice, C->stream));
@@ -724,7 +760,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
             C->part_norm.as<double>(), C->part_skipped.as<unsigned long long>(),
             C->part_taps.as<unsigned long long>(), C->part_nonfinite.as<unsigned long long>(),
             C->part_bbox.as<int4>(), C->job_out_d.as<unsigned long long>(),
-            C->job_bbox_d.as<int4>());
+            C->job_bbox_d.as<int4>(), C->job_norm_d.as<double>());
     HIP_TRY(hipGetLastError());
     record(C, EV_SELECT);
 
@@ -775,7 +811,8 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
         if (T_vis == 0)
         {
             record(C, EV_BUCKET); record(C, EV_GRID); record(C, EV_MERGE);
-            return grid_nonfinite<T>(C, num_nonfinite, num_jobs, n_total);
+            TRY(grid_nonfinite<T>(C, num_nonfinite, num_jobs, n_total));
+            return commit_jobs<T>(C, num_jobs);
         }
     }
 
@@ -795,6 +832,11 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     int cur = 0;
     TRY(radix_sort(C, T_vis, sc + SC_VISITS, key_bits, &cur));
     const unsigned* skeys = C->keys[cur].as<unsigned>();
+    // (for mi355_ctx_get_buckets: where this pass's sorted lists are)
+    C->bk_valid = strips && !C->sync_free;
+    C->bk_cur = cur;
+    C->bk_count = T_vis;
+    C->bk_key_table_bits = key_table_bits;
     const unsigned* svals = C->vals[cur].as<unsigned>();
 
     // Runs of visits sharing a tile, a kernel table and an orientation (the
@@ -992,7 +1034,8 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
         HIP_TRY(hipGetLastError());
     }
     record(C, EV_MERGE);
-    return grid_nonfinite<T>(C, num_nonfinite, num_jobs, n_total);
+    TRY(grid_nonfinite<T>(C, num_nonfinite, num_jobs, n_total));
+    return commit_jobs<T>(C, num_jobs);
 }
 
 void collect_timing(mi355_ctx* C)
@@ -1711,17 +1754,7 @@ MI355_EXPORT int mi355_ctx_destroy(mi355_ctx* C)
     if (C->capture_stream) (void)hipStreamDestroy(C->capture_stream);
     (void)hipSetDevice(C->device);
     (void)wait_ctx_work(C);
-    DevBuf* all[] = {&C->tables, &C->norm_table, &C->support_d, &C->plane_off_d, &C->meta,
-        &C->amp, &C->num_visits, &C->visit_off, &C->part_norm, &C->part_skipped,
-        &C->part_taps, &C->part_nonfinite, &C->part_bbox, &C->scan_partial, &C->scalars,
-        &C->keys[0], &C->keys[1],
-        &C->vals[0], &C->vals[1], &C->hist, &C->hist_off, &C->tile_start, &C->tile_end,
-        &C->n_items, &C->n_slots, &C->item_off, &C->slot_off, &C->item_tile, &C->run_start,
-        &C->run_z, &C->run_tb, &C->scratch,
-        &C->jobs_d, &C->job_chunk_off_d, &C->job_vis_off_d, &C->job_tile_base_d,
-        &C->job_out_d, &C->job_bbox_d, &C->in_conv, &C->in_uu, &C->in_vv, &C->in_ww,
-        &C->in_vis, &C->in_weight, &C->in_grid, &C->in_norm, &C->in_skipped, &C->in_weight_out,
-        &C->fft_work, &C->in_image, &C->vmeta, &C->vamp, &C->vlds};
+    std::vector<DevBuf*> all = device_buffers(C);
     if (C->fft_valid) (void)hipfftDestroy(C->fft_plan);
     for (DevBuf* b : all) release(C, *b);
     for (int i = 0; i < 2; ++i)
@@ -1831,6 +1864,36 @@ MI355_EXPORT int mi355_ctx_get_stats(const mi355_ctx* Cc, mi355_stats* stats)
         C->stats_deferred = false;
     }
     *stats = C->stats;
+    return ok();
+}
+
+MI355_EXPORT int mi355_ctx_get_buckets(mi355_ctx* C, unsigned long long* count,
+        int* key_table_bits, unsigned* keys, unsigned* vis, int* flags, unsigned long long cap)
+{
+    if (!C || !count) return fail(MI355_ERR_INVALID_ARG, "NULL argument");
+    if (!C->bk_valid) return fail(MI355_ERR_INVALID_ARG, "no strip-kernel gridding pass to report");
+    *count = C->bk_count;
+    if (key_table_bits) *key_table_bits = C->bk_key_table_bits;
+    if (!keys && !vis && !flags) return ok();
+    if (cap < C->bk_count) return fail(MI355_ERR_INVALID_ARG, "cap %llu < %llu visits", cap, C->bk_count);
+    HIP_TRY(hipSetDevice(C->device));
+    const size_t n = (size_t)C->bk_count;
+    if (keys)
+        HIP_TRY(hipMemcpyAsync(keys, C->keys[C->bk_cur].p, n * sizeof(unsigned),
+                hipMemcpyDeviceToHost, C->stream));
+    if (vis)
+        HIP_TRY(hipMemcpyAsync(vis, C->vals[C->bk_cur].p, n * sizeof(unsigned),
+                hipMemcpyDeviceToHost, C->stream));
+    std::vector<int4> vm;
+    if (flags)
+    {
+        vm.resize(n);
+        HIP_TRY(hipMemcpyAsync(vm.data(), C->vmeta.p, n * sizeof(int4), hipMemcpyDeviceToHost,
+                C->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    for (size_t i = 0; i < vm.size(); ++i)
+        flags[i] = ((vm[i].z & kVisitStaged) ? 1 : 0) | ((vm[i].z & kVisitRunStart) ? 2 : 0);
     return ok();
 }
 

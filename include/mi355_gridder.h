@@ -133,6 +133,19 @@ typedef struct
 
 MI355_EXPORT int mi355_ctx_get_stats(const mi355_ctx* ctx, mi355_stats* stats);
 
+/* Introspection of the last FP32 strip-kernel gridding pass on the context
+ * (not a host-synchronisation-free one): its sorted tile lists, for checking
+ * the bucketing against a restatement of the selection.  *count receives
+ * the number of (region, visibility) visits and *key_table_bits the number
+ * of low key bits below the region id; with cap >= *count, keys[i] and
+ * vis[i] receive the i-th sorted visit's key ((region << key_table_bits) |
+ * kernel-table id) and visibility index, and flags[i] bit 0 = the visit's
+ * run is staged in LDS, bit 1 = the visit starts a run (any output may be
+ * NULL).  No reference counterpart: it stands for the per-tile count and
+ * scatter of oskar_grid_wproj_gpu.cu:359-649. */
+MI355_EXPORT int mi355_ctx_get_buckets(mi355_ctx* ctx, unsigned long long* count,
+        int* key_table_bits, unsigned* keys, unsigned* vis, int* flags, unsigned long long cap);
+
 /* ------------------------------------------------------------------------
  * Device-pointer gridding: every array is in device memory except
  * `support` (host, num_w_planes ints).  num_skipped (one unsigned long long)

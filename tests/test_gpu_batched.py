@@ -179,7 +179,15 @@ def test_graph_survives_other_calls_on_the_context(gpu):
     ops.select(b["support"], b["oversample"], d["uu"], d["vv"], d["ww"], j["cell_size_rad"],
                j["w_scale"], G, ctx=ctx)
     check()
+    # the autotuner runs trials on the context and leaves its choice in the
+    # options; with the options put back, the key matches the graph built
+    # above, which must replay unchanged
+    keys = (_lib.OPT_GRID_KERNEL, _lib.OPT_VISITS_PER_ITEM, _lib.OPT_MIN_STAGE_RUN,
+            _lib.OPT_ITEM_GROUP)
+    saved = [ctx.get_option(k) for k in keys]
     ops.autotune_batched(*args, ctx=ctx)
+    for k, v in zip(keys, saved):
+        ctx.set_option(k, v)
     check()
     assert ctx.stats()["num_points"] == cat["uu"].numel()
 
