@@ -52,11 +52,22 @@ int ok()
 
 } // namespace ggs
 
+// Static device scratch of the allocation-free entry point
+// (mi355_grid_wproj_f_dev_noalloc): every device the library runs on holds
+// one copy until the process ends.  Host code never uses the name itself
+// (on the host it is a stand-in, not a device address); init_ctx resolves
+// each device's address once with hipGetSymbolAddress and the pipeline
+// carves its buffers out of that address.
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 namespace {
 
 using ggs::fail;
 using ggs::ok;
+
+std::mutex g_scratch_lock;
+void* g_scratch_addr[64] = {};       // per device: the address of `scratch`
 
 #define HIP_TRY(expr)                                                        \
     do {                                                                     \
@@ -188,10 +199,18 @@ struct mi355_ctx
     // context was a graph call when stats_in_graph is set.
     mi355_ctx* graph_ctx = nullptr;
     bool stats_in_graph = false;
+    // The allocation-free entry point runs on a context of its own whose
+    // buffers are carved, call by call, out of the device's static scratch
+    // array (arena); the last call on this context was one of those when
+    // stats_in_noalloc is set.
+    mi355_ctx* noalloc_ctx = nullptr;
+    bool stats_in_noalloc = false;
     // the last strip-kernel pass's sorted tile lists (mi355_ctx_get_buckets)
     bool bk_valid = false;
     int bk_cur = 0, bk_key_table_bits = 0;
     unsigned long long bk_count = 0;
+    unsigned char* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
 
     // prepared kernel tables
     const void* cache_conv = nullptr;
@@ -272,6 +291,20 @@ int ensure(mi355_ctx* C, DevBuf& b, size_t bytes)
 {
     if (bytes == 0) bytes = 256;
     if (bytes <= b.cap) return MI355_OK;
+    if (C->arena)
+    {
+        // allocation-free context: the next 256-byte-aligned piece of the
+        // static scratch array (pieces are handed out afresh on every call)
+        const size_t at = (C->arena_used + 255) & ~(size_t)255;
+        if (at > C->arena_cap || bytes > C->arena_cap - at)
+            return fail(MI355_ERR_TOO_LARGE,
+                    "this call needs more than the %zu bytes of static scratch the "
+                    "allocation-free entry point has (at least %zu)", C->arena_cap, at + bytes);
+        b.p = C->arena + at;
+        b.cap = bytes;
+        C->arena_used = at + bytes;
+        return MI355_OK;
+    }
     const size_t others = C->ws_bytes - b.cap;
     if (C->max_workspace && others + bytes > C->max_workspace)
         return fail(MI355_ERR_OUT_OF_MEMORY,
@@ -307,6 +340,12 @@ int ensure(mi355_ctx* C, DevBuf& b, size_t bytes)
 
 void release(mi355_ctx* C, DevBuf& b)
 {
+    if (C->arena)        // (a view into the static scratch: nothing to free)
+    {
+        b.p = nullptr;
+        b.cap = 0;
+        return;
+    }
     if (b.p) (void)hipFree(b.p);
     C->ws_bytes -= b.cap;
     b.p = nullptr;
@@ -336,6 +375,22 @@ int init_ctx(mi355_ctx* C, int device)
         C->num_cu = ncu;
     for (int i = 0; i < EV_COUNT; ++i) HIP_TRY(hipEventCreate(&C->ev[i]));
     HIP_TRY(hipEventCreateWithFlags(&C->switch_ev, hipEventDisableTiming));
+    {
+        // the device's static scratch, resolved once per device (device current)
+        std::lock_guard<std::mutex> lock(g_scratch_lock);
+        if (device < 64 && !g_scratch_addr[device])
+        {
+            void* ptr = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !ptr)
+            {
+                (void)hipGetLastError();
+                return fail(MI355_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s",
+                        hipGetErrorString(e));
+            }
+            g_scratch_addr[device] = ptr;
+        }
+    }
     return MI355_OK;
 }
 
@@ -624,6 +679,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     typedef typename V4<T>::t T4;
     C->stats = mi355_stats();
     C->stats_in_graph = false;
+    C->stats_in_noalloc = false;
     C->bk_valid = false;
     record(C, EV_START);
     TRY((prepare_tables<TI, TA>(C, P, support, ov, csh, conv)));
@@ -1749,6 +1805,8 @@ MI355_EXPORT int mi355_ctx_destroy(mi355_ctx* C)
     if (!C) return ok();
     if (C->graph_ctx) (void)mi355_ctx_destroy(C->graph_ctx);
     C->graph_ctx = nullptr;
+    if (C->noalloc_ctx) (void)mi355_ctx_destroy(C->noalloc_ctx);
+    C->noalloc_ctx = nullptr;
     if (C->graph_exec) (void)hipGraphExecDestroy(C->graph_exec);
     if (C->graph) (void)hipGraphDestroy(C->graph);
     if (C->capture_stream) (void)hipStreamDestroy(C->capture_stream);
@@ -1841,6 +1899,7 @@ MI355_EXPORT int mi355_ctx_get_stats(const mi355_ctx* Cc, mi355_stats* stats)
     if (!Cc || !stats) return fail(MI355_ERR_INVALID_ARG, "NULL argument");
     mi355_ctx* C = const_cast<mi355_ctx*>(Cc);
     if (C->stats_in_graph && C->graph_ctx) C = C->graph_ctx;
+    else if (C->stats_in_noalloc && C->noalloc_ctx) C = C->noalloc_ctx;
     if (C->stats_deferred)
     {
         // host-sync-free call: read the counts the device kept
@@ -1912,6 +1971,46 @@ MI355_EXPORT int mi355_grid_wproj_f_dev(mi355_ctx* C, size_t P, const int* suppo
 {
     if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
     return grid_dev<float, float>(C, P, support, ov, csh, conv, N, uu, vv, ww, vis, weight, cell,
+            w_scale, G, skipped, norm, grid, nullptr);
+}
+
+MI355_EXPORT int mi355_grid_wproj_f_dev_noalloc(mi355_ctx* C, size_t P, const int* support,
+        int ov, int csh, const float* conv, size_t N, const float* uu, const float* vv,
+        const float* ww, const float* vis, const float* weight, float cell, float w_scale,
+        int G, unsigned long long* skipped, double* norm, float* grid)
+{
+    if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");
+    HIP_TRY(hipSetDevice(C->device));
+    // Set-up on the first call: the context's allocation-free twin, whose
+    // buffers are pieces of this device's static scratch (its address was
+    // resolved when the context was created).
+    if (!C->noalloc_ctx)
+    {
+        if (C->device >= 64 || !g_scratch_addr[C->device])
+            return fail(MI355_ERR_HIP, "no static scratch address for device %d", C->device);
+        mi355_ctx* Z = new mi355_ctx();
+        const int st = init_ctx(Z, C->device);
+        if (st != MI355_OK) { delete Z; return st; }
+        Z->arena = static_cast<unsigned char*>(g_scratch_addr[C->device]);
+        Z->arena_cap = kNoAllocBytes;
+        C->noalloc_ctx = Z;
+    }
+    mi355_ctx* Z = C->noalloc_ctx;
+    Z->visits_per_item = C->visits_per_item;
+    Z->compensated = C->compensated;
+    Z->grid_kernel = C->grid_kernel;
+    Z->min_stage_run = C->min_stage_run;
+    Z->item_group = C->item_group;
+    Z->timing = C->timing;
+    Z->cache_kernels = false;          // (the scratch is shared: tables are rebuilt per call)
+    Z->tables_valid = false;
+    if (Z->stream != C->stream) TRY(mi355_ctx_set_stream(Z, C->stream));
+    // every buffer is handed out afresh from the start of the scratch
+    Z->arena_used = 0;
+    for (DevBuf* b : device_buffers(Z)) { b->p = nullptr; b->cap = 0; }
+    C->stats_in_noalloc = true;
+    C->stats_in_graph = false;
+    return grid_dev<float, float>(Z, P, support, ov, csh, conv, N, uu, vv, ww, vis, weight, cell,
             w_scale, G, skipped, norm, grid, nullptr);
 }
 

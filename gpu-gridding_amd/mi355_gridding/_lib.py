@@ -134,6 +134,7 @@ SIGNATURES = {
     "mi355_ctx_get_buckets": (I, [P, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(I), P, P,
                                   P, ctypes.c_ulonglong]),
     "mi355_grid_wproj_f_dev": (I, [P, SZ, P, I, I, P, SZ, P, P, P, P, P, F, F, I, P, P, P]),
+    "mi355_grid_wproj_f_dev_noalloc": (I, [P, SZ, P, I, I, P, SZ, P, P, P, P, P, F, F, I, P, P, P]),
     "mi355_grid_wproj_d_dev": (I, [P, SZ, P, I, I, P, SZ, P, P, P, P, P, D, D, I, P, P, P]),
     "mi355_grid_wproj_fsel_d_dev": (I, [P, SZ, P, I, I, P, SZ, P, P, P, P, P, F, F, I, P, P, P]),
     "mi355_select_wproj_f_dev": (I, [P, SZ, P, I, SZ, P, P, P, F, F, I, P]),

@@ -160,6 +160,27 @@ MI355_EXPORT int mi355_grid_wproj_f_dev(mi355_ctx* ctx,
         int grid_size, unsigned long long* num_skipped, double* norm,
         float* grid);
 
+/* The same gridding pass without any device allocation.  No reference
+ * counterpart: for callers that cannot allocate device memory during the
+ * call (work captured into a HIP graph, tight serving loops, harnesses that
+ * forbid hipMalloc).  Every buffer the pass needs beyond the caller's inputs
+ * and outputs is a piece of a static device array of 64 MiB that each device
+ * the library runs on holds until the process ends (its address is resolved
+ * when a context is created; the first call on a context also creates its
+ * events).  A call that needs more fails with MI355_ERR_TOO_LARGE before its
+ * outputs are touched.  All calls on one device share that array, and the
+ * array keeps the last call's contents: calls that overlap on one device
+ * (on any contexts or streams) race.  The kernel tables are rebuilt on every
+ * call (MI355_OPT_CACHE_KERNELS does not apply); the other options are the
+ * context's. */
+MI355_EXPORT int mi355_grid_wproj_f_dev_noalloc(mi355_ctx* ctx,
+        size_t num_w_planes, const int* support, int oversample,
+        int conv_size_half, const float* conv_func, size_t num_points,
+        const float* uu, const float* vv, const float* ww, const float* vis,
+        const float* weight, float cell_size_rad, float w_scale,
+        int grid_size, unsigned long long* num_skipped, double* norm,
+        float* grid);
+
 MI355_EXPORT int mi355_grid_wproj_d_dev(mi355_ctx* ctx,
         size_t num_w_planes, const int* support, int oversample,
         int conv_size_half, const double* conv_func, size_t num_points,
