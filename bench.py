@@ -417,14 +417,45 @@ def bench_driver():
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     wall, res = min(runs, key=lambda r: r[0])
-    return dict(seconds_total=round(wall, 4), seconds_read=round(res["seconds_read"], 4),
+    # one block's weights read and gridding on resident inputs, for
+    # comparison with the driver's per-block sum (blocks x this)
+    import torch
+    from mi355_gridding import torch_ops as ops
+    n = B * tpb
+    ks = ops.KernelStack(c["support"], c["oversample"], c["conv_size_half"], c["conv_func"])
+    d = {k: torch.from_numpy(np.ascontiguousarray(c[k][:n])).cuda()
+         for k in ("uu", "vv", "ww", "vis", "weight")}
+    G = c["grid_size"]
+    grid = torch.zeros((G, G, 2), dtype=torch.float32, device="cuda")
+    nrm = torch.zeros(1, dtype=torch.float64, device="cuda")
+    skp = torch.zeros(1, dtype=torch.int64, device="cuda")
+    call = lambda: ops.grid(ks, d["uu"], d["vv"], d["ww"], d["vis"], d["weight"],
+                            c["cell_size_rad"], c["w_scale"], G, grid, nrm, skp)
+    for _ in range(3):
+        call()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        call()
+    e1.record()
+    torch.cuda.synchronize()
+    block_ms = e0.elapsed_time(e1) / 10
+    return dict(seconds_total=round(wall, 4), seconds_wall=round(res["seconds_wall"], 4),
+                seconds_read=round(res["seconds_read"], 4),
                 seconds_weights=round(res["seconds_weights"], 4),
                 seconds_grid=round(res["seconds_grid"], 4), num_blocks=res["num_blocks"],
                 num_points=res["num_points"],
+                block_grid_ms=round(block_ms, 3),
+                grid_vs_blocks_x_block=round(res["seconds_grid"] * 1e3 /
+                                             (res["num_blocks"] * block_ms), 3),
                 mvis_s_grid=round(res["num_points"] / res["seconds_grid"] / 1e6, 2),
                 mvis_s_total=round(res["num_points"] / wall / 1e6, 2),
                 note="mi355_grid_vis_file over a .vis file and FITS kernel cubes in a scratch "
-                     "directory (file reads, weights and gridding of every block; best of 2)")
+                     "directory (file reads, weights and gridding of every block; best of 2); "
+                     "seconds_grid sums each block's device time from its copies to its last "
+                     "pass; block_grid_ms is one %d-visibility block gridded on resident "
+                     "inputs (mean of 10)" % n)
 
 
 def main():

@@ -4,6 +4,7 @@
 // stack, weights grid, visibility grid, norm) resident on the device.
 // Built on the library's public C ABI (mi355_gridder.h, mi355_io.h).
 #include "gg_status.hpp"
+#include "mi355_gridder.h"
 #include "mi355_io.h"
 
 #include <hip/hip_runtime.h>
@@ -148,6 +149,55 @@ struct ApiD
     { return mi355_grid_to_image_d_dev(c, G, g, nrm, img); }
 };
 
+// Pinned host allocations of one run, released on every return path.
+struct HostArrays
+{
+    std::vector<void*> ptrs;
+    ~HostArrays() { for (void* p : ptrs) (void)hipHostFree(p); }
+    int alloc(void** p, size_t bytes)
+    {
+        *p = nullptr;
+        hipError_t e = hipHostMalloc(p, bytes ? bytes : 16, hipHostMallocDefault);
+        if (e != hipSuccess)
+        {
+            (void)hipGetLastError();
+            return fail(MI355_ERR_OUT_OF_MEMORY, "pinned host allocation of %zu bytes failed", bytes);
+        }
+        ptrs.push_back(*p);
+        return MI355_OK;
+    }
+};
+
+// Events of one run, destroyed on every return path.
+struct Events
+{
+    std::vector<hipEvent_t> ev;
+    ~Events() { for (hipEvent_t e : ev) (void)hipEventDestroy(e); }
+    int make(hipEvent_t* e, bool timing)
+    {
+        hipError_t r = hipEventCreateWithFlags(e, timing ? hipEventDefault : hipEventDisableTiming);
+        if (r != hipSuccess)
+            return fail(MI355_ERR_HIP, "hipEventCreateWithFlags failed: %s", hipGetErrorString(r));
+        ev.push_back(*e);
+        return MI355_OK;
+    }
+};
+
+// Waits for the run's stream when it goes out of scope, so that no copy or
+// kernel still uses the buffers that are released after it (error paths).
+struct StreamDrain
+{
+    hipStream_t s;
+    ~StreamDrain() { (void)hipStreamSynchronize(s); }
+};
+
+// The run, ordered on the context's stream.  Blocks go through two sets of
+// pinned host and device input buffers: block b+1 is read from the file
+// while block b's copies and passes run; a set is refilled once the event
+// recorded after its copies has passed.  Every block's num_skipped goes to
+// a slot of its own on the device, read back once at the end, so nothing in
+// the block loops waits for the device (the gridding call itself waits for
+// its selection to size its buffers).
 template <typename Api>
 int run(mi355_ctx* ctx, Files& F, const mi355_vis_header& hd, const mi355_kernel_info& ki,
         int check, void* grid_host, size_t grid_capacity, const char* image_path,
@@ -160,118 +210,179 @@ int run(mi355_ctx* ctx, Files& F, const mi355_vis_header& hd, const mi355_kernel
     const T cell = (T)ki.cellsize_rad, w_scale = (T)ki.w_scale;
     const size_t cap = (size_t)hd.max_times_per_block * (size_t)hd.num_baselines;
     const size_t cells = (size_t)G * G;
+    const int nb = hd.num_blocks;
+    const auto t_start = std::chrono::steady_clock::now();
+    hipStream_t st = nullptr;
+    DRV_TRY(mi355_ctx_get_stream(ctx, reinterpret_cast<void**>(&st)));
 
+    DevArrays D;
+    HostArrays H;
+    Events E;
+    void *d_conv, *d_w1, *d_w, *d_wgrid, *d_grid, *d_norm, *d_sk;
+    void *d_uu[2], *d_vv[2], *d_ww[2], *d_vis[2];
+    void *h_conv, *h_uu[2], *h_vv[2], *h_ww[2], *h_vis[2], *h_sk, *h_norm;
     // kernel stack: real and imaginary cubes interleaved (main.cpp:186-199)
     const size_t nk = (size_t)csh * csh * P;
-    std::vector<T> conv(2 * nk);
+    DRV_TRY(H.alloc(&h_conv, 2 * nk * sizeof(T)));
     for (size_t j = 0; j < nk; ++j)
     {
-        conv[2 * j] = static_cast<const T*>(F.kre)[j];
-        conv[2 * j + 1] = static_cast<const T*>(F.kim)[j];
+        static_cast<T*>(h_conv)[2 * j] = static_cast<const T*>(F.kre)[j];
+        static_cast<T*>(h_conv)[2 * j + 1] = static_cast<const T*>(F.kim)[j];
     }
-    DevArrays D;
-    void *d_conv, *d_uu, *d_vv, *d_ww, *d_vis, *d_w1, *d_w, *d_wgrid, *d_grid, *d_norm, *d_sk;
-    DRV_TRY(D.alloc(&d_conv, conv.size() * sizeof(T)));
-    DRV_TRY(D.alloc(&d_uu, cap * sizeof(T)));
-    DRV_TRY(D.alloc(&d_vv, cap * sizeof(T)));
-    DRV_TRY(D.alloc(&d_ww, cap * sizeof(T)));
-    DRV_TRY(D.alloc(&d_vis, 2 * cap * sizeof(T)));
+    // num_skipped slots: pass 1 per block, pass 2 per block, the FP64 check
+    // per block, one for the weights read
+    const size_t n_sk = 3 * (size_t)nb + 1;
+    DRV_TRY(H.alloc(&h_sk, n_sk * sizeof(unsigned long long)));
+    DRV_TRY(H.alloc(&h_norm, 2 * sizeof(double)));
+    DRV_TRY(D.alloc(&d_conv, 2 * nk * sizeof(T)));
+    for (int k = 0; k < 2; ++k)
+    {
+        DRV_TRY(D.alloc(&d_uu[k], cap * sizeof(T)));
+        DRV_TRY(D.alloc(&d_vv[k], cap * sizeof(T)));
+        DRV_TRY(D.alloc(&d_ww[k], cap * sizeof(T)));
+        DRV_TRY(D.alloc(&d_vis[k], 2 * cap * sizeof(T)));
+        DRV_TRY(H.alloc(&h_uu[k], cap * sizeof(T)));
+        DRV_TRY(H.alloc(&h_vv[k], cap * sizeof(T)));
+        DRV_TRY(H.alloc(&h_ww[k], cap * sizeof(T)));
+        DRV_TRY(H.alloc(&h_vis[k], 2 * cap * sizeof(T)));
+    }
     DRV_TRY(D.alloc(&d_w1, cap * sizeof(T)));
     DRV_TRY(D.alloc(&d_w, cap * sizeof(T)));
     DRV_TRY(D.alloc(&d_wgrid, cells * sizeof(T)));
     DRV_TRY(D.alloc(&d_grid, 2 * cells * sizeof(T)));
     DRV_TRY(D.alloc(&d_norm, 2 * sizeof(double)));
-    DRV_TRY(D.alloc(&d_sk, 2 * sizeof(unsigned long long)));
+    DRV_TRY(D.alloc(&d_sk, n_sk * sizeof(unsigned long long)));
     void* d_grid64 = nullptr;
     if (check) DRV_TRY(D.alloc(&d_grid64, 2 * cells * sizeof(double)));
-    DRV_HIP(hipMemcpy(d_conv, conv.data(), conv.size() * sizeof(T), hipMemcpyHostToDevice));
-    DRV_HIP(hipMemset(d_wgrid, 0, cells * sizeof(T)));
-    DRV_HIP(hipMemset(d_grid, 0, 2 * cells * sizeof(T)));
-    DRV_HIP(hipMemset(d_norm, 0, 2 * sizeof(double)));
-    DRV_HIP(hipMemset(d_w, 0, cap * sizeof(T)));     // (main.cpp callocs it, :185)
-    if (check) DRV_HIP(hipMemset(d_grid64, 0, 2 * cells * sizeof(double)));
+    hipEvent_t ev_free[2];
+    bool pending[2] = {false, false};
+    std::vector<hipEvent_t> ev_t(4 * (size_t)nb);   // per block and pass: start, end
+    for (int k = 0; k < 2; ++k) DRV_TRY(E.make(&ev_free[k], false));
+    for (auto& e : ev_t) DRV_TRY(E.make(&e, true));
+    StreamDrain drain{st};
+
+    DRV_HIP(hipMemcpyAsync(d_conv, h_conv, 2 * nk * sizeof(T), hipMemcpyHostToDevice, st));
+    DRV_HIP(hipMemsetAsync(d_wgrid, 0, cells * sizeof(T), st));
+    DRV_HIP(hipMemsetAsync(d_grid, 0, 2 * cells * sizeof(T), st));
+    DRV_HIP(hipMemsetAsync(d_norm, 0, 2 * sizeof(double), st));
+    DRV_HIP(hipMemsetAsync(d_sk, 0, n_sk * sizeof(unsigned long long), st));
+    DRV_HIP(hipMemsetAsync(d_w, 0, cap * sizeof(T), st));     // (main.cpp callocs it, :185)
+    if (check) DRV_HIP(hipMemsetAsync(d_grid64, 0, 2 * cells * sizeof(double), st));
     // unit input weights (main.cpp:195-205)
-    hipLaunchKernelGGL(k_fill_ones, dim3(256), dim3(256), 0, 0, d_w1, (unsigned long long)cap,
+    hipLaunchKernelGGL(k_fill_ones, dim3(256), dim3(256), 0, st, d_w1, (unsigned long long)cap,
             (int)(sizeof(T) == 8));
     DRV_HIP(hipGetLastError());
-    DRV_HIP(hipDeviceSynchronize());      // (the context's stream may be another)
 
-    std::vector<T> uu(cap), vv(cap), ww(cap), vis(2 * cap);
-    double t_read = 0, t_weights = 0, t_grid = 0;
-    unsigned long long points = 0, skipped_total = 0, skipped_last = 0;
+    double t_read = 0;
+    unsigned long long points = 0;
+    size_t n_in[2] = {0, 0};
+    // block b into host set k (once the set's previous copies are done)
+    auto read = [&](int b, int k, bool amps) -> int {
+        if (pending[k])
+        {
+            DRV_HIP(hipEventSynchronize(ev_free[k]));
+            pending[k] = false;
+        }
+        const auto t0 = std::chrono::steady_clock::now();
+        DRV_TRY(mi355_vis_read_block(F.vis, &hd, b, cap, &n_in[k], (T*)h_uu[k], (T*)h_vv[k],
+                (T*)h_ww[k], amps ? (T*)h_vis[k] : nullptr));
+        t_read += seconds_since(t0);
+        return MI355_OK;
+    };
+    auto upload = [&](int k, bool amps) -> int {
+        const size_t n = n_in[k];
+        DRV_HIP(hipMemcpyAsync(d_uu[k], h_uu[k], n * sizeof(T), hipMemcpyHostToDevice, st));
+        DRV_HIP(hipMemcpyAsync(d_vv[k], h_vv[k], n * sizeof(T), hipMemcpyHostToDevice, st));
+        if (amps)
+        {
+            DRV_HIP(hipMemcpyAsync(d_ww[k], h_ww[k], n * sizeof(T), hipMemcpyHostToDevice, st));
+            DRV_HIP(hipMemcpyAsync(d_vis[k], h_vis[k], 2 * n * sizeof(T), hipMemcpyHostToDevice, st));
+        }
+        DRV_HIP(hipEventRecord(ev_free[k], st));
+        pending[k] = true;
+        return MI355_OK;
+    };
+    unsigned long long* sk = static_cast<unsigned long long*>(d_sk);
 
     // Pass 1: the uniform-weighting grid over every block (main.cpp:202-239).
-    for (int b = 0; b < hd.num_blocks; ++b)
+    if (nb > 0) DRV_TRY(read(0, 0, false));
+    for (int b = 0; b < nb; ++b)
     {
-        auto t0 = std::chrono::steady_clock::now();
-        size_t n = 0;
-        DRV_TRY(mi355_vis_read_block(F.vis, &hd, b, cap, &n, uu.data(), vv.data(), ww.data(), nullptr));
-        t_read += seconds_since(t0);
-        t0 = std::chrono::steady_clock::now();
-        DRV_HIP(hipMemcpy(d_uu, uu.data(), n * sizeof(T), hipMemcpyHostToDevice));
-        DRV_HIP(hipMemcpy(d_vv, vv.data(), n * sizeof(T), hipMemcpyHostToDevice));
-        DRV_TRY(Api::wwrite(ctx, n, (const T*)d_uu, (const T*)d_vv, (const T*)d_w1, cell, G,
-                (unsigned long long*)d_sk, (T*)d_wgrid));
-        DRV_HIP(hipDeviceSynchronize());
-        t_weights += seconds_since(t0);
+        const int k = b & 1;
+        const size_t n = n_in[k];
+        DRV_HIP(hipEventRecord(ev_t[4 * b], st));
+        DRV_TRY(upload(k, false));
+        DRV_TRY(Api::wwrite(ctx, n, (const T*)d_uu[k], (const T*)d_vv[k], (const T*)d_w1, cell, G,
+                sk + b, (T*)d_wgrid));
+        DRV_HIP(hipEventRecord(ev_t[4 * b + 1], st));
+        if (b + 1 < nb) DRV_TRY(read(b + 1, k ^ 1, false));
     }
 
     // Pass 2: weights read and gridding, block by block (main.cpp:242-381).
-    for (int b = 0; b < hd.num_blocks; ++b)
+    if (nb > 0) DRV_TRY(read(0, nb & 1, true));
+    for (int b = 0; b < nb; ++b)
     {
-        auto t0 = std::chrono::steady_clock::now();
-        size_t n = 0;
-        DRV_TRY(mi355_vis_read_block(F.vis, &hd, b, cap, &n, uu.data(), vv.data(), ww.data(), vis.data()));
-        t_read += seconds_since(t0);
-        t0 = std::chrono::steady_clock::now();
-        DRV_HIP(hipMemcpy(d_uu, uu.data(), n * sizeof(T), hipMemcpyHostToDevice));
-        DRV_HIP(hipMemcpy(d_vv, vv.data(), n * sizeof(T), hipMemcpyHostToDevice));
-        DRV_HIP(hipMemcpy(d_ww, ww.data(), n * sizeof(T), hipMemcpyHostToDevice));
-        DRV_HIP(hipMemcpy(d_vis, vis.data(), 2 * n * sizeof(T), hipMemcpyHostToDevice));
-        DRV_TRY(Api::wread(ctx, n, (const T*)d_uu, (const T*)d_vv, (const T*)d_w1, (T*)d_w, cell, G,
-                (unsigned long long*)d_sk, (const T*)d_wgrid));
+        const int k = (b + nb) & 1;
+        const size_t n = n_in[k];
+        points += n;
+        DRV_HIP(hipEventRecord(ev_t[4 * b + 2], st));
+        DRV_TRY(upload(k, true));
+        DRV_TRY(Api::wread(ctx, n, (const T*)d_uu[k], (const T*)d_vv[k], (const T*)d_w1, (T*)d_w,
+                cell, G, sk + 3 * nb, (const T*)d_wgrid));
         DRV_TRY(Api::grid(ctx, (size_t)P, F.support, ki.oversample, csh, (const T*)d_conv, n,
-                (const T*)d_uu, (const T*)d_vv, (const T*)d_ww, (const T*)d_vis, (const T*)d_w,
-                cell, w_scale, G, (unsigned long long*)d_sk, (double*)d_norm, (T*)d_grid));
+                (const T*)d_uu[k], (const T*)d_vv[k], (const T*)d_ww[k], (const T*)d_vis[k],
+                (const T*)d_w, cell, w_scale, G, sk + nb + b, (double*)d_norm, (T*)d_grid));
         if (check)
             DRV_TRY(Api::grid64(ctx, (size_t)P, F.support, ki.oversample, csh, (const T*)d_conv, n,
-                    (const T*)d_uu, (const T*)d_vv, (const T*)d_ww, (const T*)d_vis, (const T*)d_w,
-                    cell, w_scale, G, (unsigned long long*)d_sk + 1, (double*)d_norm + 1,
+                    (const T*)d_uu[k], (const T*)d_vv[k], (const T*)d_ww[k], (const T*)d_vis[k],
+                    (const T*)d_w, cell, w_scale, G, sk + 2 * nb + b, (double*)d_norm + 1,
                     (double*)d_grid64));
-        DRV_HIP(hipDeviceSynchronize());
-        t_grid += seconds_since(t0);
-        unsigned long long sk = 0;
-        DRV_HIP(hipMemcpy(&sk, d_sk, sizeof(sk), hipMemcpyDeviceToHost));
-        skipped_last = sk;
-        skipped_total += sk;
-        points += n;
+        DRV_HIP(hipEventRecord(ev_t[4 * b + 3], st));
+        if (b + 1 < nb) DRV_TRY(read(b + 1, k ^ 1, true));
     }
-    double norm[2] = {0, 0};
-    DRV_HIP(hipMemcpy(norm, d_norm, sizeof(norm), hipMemcpyDeviceToHost));
+    DRV_HIP(hipMemcpyAsync(h_sk, d_sk, n_sk * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    DRV_HIP(hipMemcpyAsync(h_norm, d_norm, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    DRV_HIP(hipStreamSynchronize(st));
+    const double t_wall = seconds_since(t_start);
+    double t_weights = 0, t_grid = 0;
+    unsigned long long skipped_total = 0;
+    for (int b = 0; b < nb; ++b)
+    {
+        float ms1 = 0, ms2 = 0;
+        DRV_HIP(hipEventElapsedTime(&ms1, ev_t[4 * b], ev_t[4 * b + 1]));
+        DRV_HIP(hipEventElapsedTime(&ms2, ev_t[4 * b + 2], ev_t[4 * b + 3]));
+        t_weights += 1e-3 * ms1;
+        t_grid += 1e-3 * ms2;
+        skipped_total += static_cast<unsigned long long*>(h_sk)[nb + b];
+    }
+    const double* norm = static_cast<const double*>(h_norm);
 
-    res->num_blocks = hd.num_blocks;
+    res->num_blocks = nb;
     res->grid_size = G;
     res->num_w_planes = P;
     res->precision = sizeof(T) == 8 ? MI355_BIN_DOUBLE : MI355_BIN_SINGLE;
     res->num_points = points;
     res->num_skipped_total = skipped_total;
-    res->num_skipped_last = skipped_last;
+    res->num_skipped_last = nb > 0 ? static_cast<unsigned long long*>(h_sk)[2 * nb - 1] : 0;
     res->norm = norm[0];
     res->seconds_read = t_read;
     res->seconds_weights = t_weights;
     res->seconds_grid = t_grid;
+    res->seconds_wall = t_wall;
     res->check_max_rel = res->check_l2_rel = -1.0;
     if (check)
     {
         const unsigned blocks = 1024;
-        void* d_part;
+        void *d_part, *h_part;
         DRV_TRY(D.alloc(&d_part, 4 * blocks * sizeof(double)));
-        hipLaunchKernelGGL((k_grid_error<T, double>), dim3(blocks), dim3(256), 0, 0,
+        DRV_TRY(H.alloc(&h_part, 4 * blocks * sizeof(double)));
+        hipLaunchKernelGGL((k_grid_error<T, double>), dim3(blocks), dim3(256), 0, st,
                 (const T*)d_grid, (const double*)d_grid64, (unsigned long long)(2 * cells),
                 (double*)d_part);
         DRV_HIP(hipGetLastError());
-        std::vector<double> part(4 * blocks);
-        DRV_HIP(hipMemcpy(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost));
+        DRV_HIP(hipMemcpyAsync(h_part, d_part, 4 * blocks * sizeof(double), hipMemcpyDeviceToHost, st));
+        DRV_HIP(hipStreamSynchronize(st));
+        const double* part = static_cast<const double*>(h_part);
         double md = 0, mb = 0, sd = 0, sb = 0;
         for (unsigned k = 0; k < blocks; ++k)
         {
@@ -285,7 +396,8 @@ int run(mi355_ctx* ctx, Files& F, const mi355_vis_header& hd, const mi355_kernel
     {
         if (grid_capacity < cells)
             return fail(MI355_ERR_INVALID_ARG, "grid holds %zu cells, need %zu", grid_capacity, cells);
-        DRV_HIP(hipMemcpy(grid_host, d_grid, 2 * cells * sizeof(T), hipMemcpyDeviceToHost));
+        DRV_HIP(hipMemcpyAsync(grid_host, d_grid, 2 * cells * sizeof(T), hipMemcpyDeviceToHost, st));
+        DRV_HIP(hipStreamSynchronize(st));
     }
     if (image_path)
     {
@@ -293,8 +405,8 @@ int run(mi355_ctx* ctx, Files& F, const mi355_vis_header& hd, const mi355_kernel
         DRV_TRY(D.alloc(&d_img, cells * sizeof(T)));
         DRV_TRY(Api::image(ctx, G, (const T*)d_grid, norm[0], (T*)d_img));
         std::vector<T> img(cells);
-        DRV_HIP(hipDeviceSynchronize());
-        DRV_HIP(hipMemcpy(img.data(), d_img, cells * sizeof(T), hipMemcpyDeviceToHost));
+        DRV_HIP(hipMemcpyAsync(img.data(), d_img, cells * sizeof(T), hipMemcpyDeviceToHost, st));
+        DRV_HIP(hipStreamSynchronize(st));
         DRV_TRY(mi355_fits_write_image(image_path, res->precision, G, G, img.data()));
     }
     return ok();

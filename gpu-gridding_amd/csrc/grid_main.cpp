@@ -54,6 +54,7 @@ int main(int argc, char** argv)
     printf("Reading the file took %.3f seconds\n", r.seconds_read);
     printf("Generating weights took %.3f seconds (GPU, copies included)\n", r.seconds_weights);
     printf("Gridding visibilities took %.3f seconds (GPU, copies included)\n", r.seconds_grid);
+    printf("Both passes took %.3f seconds (wall clock, file reads included)\n", r.seconds_wall);
     if (image) printf("Image written to '%s'\n", image);
     mi355_ctx_destroy(ctx);
     return check && !(r.check_max_rel <= 1e-5) ? EXIT_FAILURE : EXIT_SUCCESS;

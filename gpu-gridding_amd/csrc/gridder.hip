@@ -1843,6 +1843,13 @@ MI355_EXPORT int mi355_ctx_set_stream(mi355_ctx* C, void* stream)
     return ok();
 }
 
+MI355_EXPORT int mi355_ctx_get_stream(mi355_ctx* C, void** stream)
+{
+    if (!C || !stream) return fail(MI355_ERR_INVALID_ARG, "NULL argument");
+    *stream = reinterpret_cast<void*>(C->stream);
+    return ok();
+}
+
 MI355_EXPORT int mi355_ctx_set_option(mi355_ctx* C, int key, long long value)
 {
     if (!C) return fail(MI355_ERR_INVALID_ARG, "ctx is NULL");

@@ -105,7 +105,7 @@ class DriverResult(ctypes.Structure):
                 ("num_skipped_last", ctypes.c_ulonglong), ("norm", ctypes.c_double),
                 ("seconds_read", ctypes.c_double), ("seconds_weights", ctypes.c_double),
                 ("seconds_grid", ctypes.c_double), ("check_max_rel", ctypes.c_double),
-                ("check_l2_rel", ctypes.c_double)]
+                ("check_l2_rel", ctypes.c_double), ("seconds_wall", ctypes.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -128,6 +128,7 @@ SIGNATURES = {
     "mi355_ctx_destroy": (I, [P]),
     "mi355_default_ctx": (I, [ctypes.POINTER(P)]),
     "mi355_ctx_set_stream": (I, [P, P]),
+    "mi355_ctx_get_stream": (I, [P, ctypes.POINTER(P)]),
     "mi355_ctx_set_option": (I, [P, I, ctypes.c_longlong]),
     "mi355_ctx_get_option": (I, [P, I, ctypes.POINTER(ctypes.c_longlong)]),
     "mi355_ctx_get_stats": (I, [P, ctypes.POINTER(Stats)]),

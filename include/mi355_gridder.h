@@ -91,6 +91,8 @@ MI355_EXPORT int mi355_ctx_destroy(mi355_ctx* ctx);
 MI355_EXPORT int mi355_default_ctx(mi355_ctx** ctx);
 /* Work is enqueued on this HIP stream (a hipStream_t; 0 = null stream). */
 MI355_EXPORT int mi355_ctx_set_stream(mi355_ctx* ctx, void* hip_stream);
+/* The stream the context's work is enqueued on (as set above; 0 = null). */
+MI355_EXPORT int mi355_ctx_get_stream(mi355_ctx* ctx, void** hip_stream);
 
 /* Tuning and behaviour options (mi355_ctx_set_option). */
 #define MI355_OPT_VISITS_PER_ITEM 1  /* tile visits per work item (default 4096) */

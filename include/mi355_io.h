@@ -171,9 +171,12 @@ MI355_EXPORT void mi355_io_free(void* p);
  * device.  The amplitude precision picks FP32 or FP64 throughout, as in
  * main.cpp.  With check set, every block is also gridded with FP64
  * accumulation (FP32 data: mi355_grid_wproj_fsel_d_dev) and the grid's
- * norm-wise error against it is reported.  The driver runs on the null
- * stream and synchronises the device after each block; the context's
- * options (kernel choice, item size ...) apply.
+ * norm-wise error against it is reported.  Everything runs in order on
+ * the context's stream (mi355_ctx_set_stream): blocks are read into two
+ * alternating pinned buffers, block b+1 from the file while block b's
+ * copies and passes run, and the per-block num_skipped stay on the device
+ * until the end, so the loops do not wait for the device between blocks.
+ * The context's options (kernel choice, item size ...) apply.
  * ---------------------------------------------------------------------- */
 typedef struct mi355_driver_result
 {
@@ -182,8 +185,12 @@ typedef struct mi355_driver_result
     unsigned long long num_skipped_total;                 /* over all blocks */
     unsigned long long num_skipped_last;                  /* last block's, as main.cpp reports */
     double norm;
-    double seconds_read, seconds_weights, seconds_grid;    /* host I/O, GPU passes */
+    double seconds_read;                                  /* host file reads */
+    double seconds_weights, seconds_grid;                 /* per pass: sum over blocks of the
+                                                           * device time from a block's
+                                                           * copies to its last pass (events) */
     double check_max_rel, check_l2_rel;                   /* check only (else -1) */
+    double seconds_wall;                                  /* both passes, reads included */
 } mi355_driver_result;
 
 /* grid: host array of grid_size^2 complex values in the amplitude precision
