@@ -18,12 +18,12 @@
 // raw buffer loads.
 //
 // Rows go in chunks of eight (two slots).  A visit touches chunks q0..q1 of
-// the strip; a chunk is skipped with a wave-uniform branch.  From LDS the
-// four slots of a pair of chunks are read together before their
-// multiplies; from global memory the touched chunks are requested
-// kDirectGroup at a time.  Rows of a touched chunk outside the footprint
-// read zero rows, columns outside it zero columns (staged) or nothing
-// (direct: the buffer offset is out of range).
+// the strip.  From LDS it takes the pairs of chunks (16 rows, four slots)
+// that hold them, the four slots of a pair read together and all four
+// multiplied; from global memory the touched chunks are requested
+// kDirectGroup at a time, others skipped with a wave-uniform branch.  Rows
+// outside the footprint read zero rows, columns outside it zero columns
+// (staged) or nothing (direct: the buffer offset is out of range).
 //
 // Each wave walks all visits of a run segment 64 at a time: one vector load
 // per lane of the records, a ballot of the visits whose footprint meets its
@@ -92,6 +92,8 @@ constexpr int kStripMaxW = 111;   // widest table staged
 #define GG_STRIP_PREFETCH 1
 #endif
 static_assert(kStripGuard - 8 * kStripS - 15 >= 0, "rows -8.. must stay inside the LDS array");
+static_assert(kStripGuard + (kStripMaxW + 2 * kStripPad + 7) * kStripS + 128 <= kStripLds,
+              "the zero rows after the widest staged table must stay inside the LDS array");
 static_assert(kStripGuard + (kStripMaxW + 21) * kStripS + kStripMaxW + 15 < kStripLds,
               "reads of the widest staged table must stay inside the LDS array");
 static_assert(kStripS - 15 >= kStripMaxW, "columns -15..-1 must land on zero columns");
@@ -117,20 +119,28 @@ __device__ __forceinline__ X rb_x4(__amdgpu_buffer_rsrc_t rs, unsigned off)
 
 // Copy a table into LDS at stride kStripS: staged row t + kStripPad,
 // column c holds table row t, column c, for t = -7 .. W+6 (the pad rows of
-// the padded block are zero); columns W .. 127 are zero and columns 128 ..
-// kStripS-1 stay zero from the kernel's start (no staged table reaches
-// them).  Visits of all four orientations read this one copy: mirrored
-// columns and rows are in their LDS addresses.
+// the padded block are zero), and staged rows -8 .. -1 and W+14 .. W+21
+// (table rows -15 .. -8 and W+7 .. W+14, which the pair of chunks around a
+// footprint's first or last row reaches) are cleared; columns W .. 127 are
+// zero and columns 128 .. kStripS-1 stay zero from the kernel's start (no
+// staged table reaches them).  Visits of all four orientations read this
+// one copy: mirrored columns and rows are in their LDS addresses.
 //
 // The copy goes straight from memory to LDS (buffer loads with the LDS
 // flag): no registers hold the data, so every row's loads are issued at
-// once and a table costs one round trip to memory.  Wave w copies rows w,
-// w + WAVES, ...; one 16-byte load per lane and row moves columns
-// 2l, 2l+1 to LDS (columns 0 .. 127 per row).  Lanes with 2l >= W load
-// nothing and write zeros; lane s (2s = W-1) also brings the tap after the
-// row's last (the next row's first), which one zero store per row then
-// clears.  The caller's barrier after the copy orders it before any read.
+// once and a table costs one round trip to memory.  Wave w copies staged
+// rows w-8, w-8 + WAVES, ...; one 16-byte load per lane and row moves
+// columns 2l, 2l+1 to LDS (columns 0 .. 127 per row).  Lanes with 2l >= W,
+// and every lane of the zero rows, load nothing (the offset is out of the
+// buffer's range) and write zeros; lane s (2s = W-1) also brings the tap
+// after the row's last (the next row's first), which one zero store per
+// row then clears.  The caller's barrier after the copy orders it before
+// any read.
 typedef __attribute__((address_space(3))) void lds_void_t;
+// s_waitcnt immediate (gfx9 encoding) for vmcnt(0) with expcnt and lgkmcnt
+// left at their maxima: wait until this wave's vector memory loads,
+// direct-to-LDS copies included, have completed.
+constexpr int kWaitVmcnt0 = (7 << 4) | (15 << 8);
 static_assert(kStripMaxW < 128, "staged columns 128 .. kStripS-1 are always zero");
 static_assert((kStripGuard * 8) % 16 == 0 && (kStripS * 8) % 16 == 0,
               "staged rows must start on 16 bytes");
@@ -141,16 +151,22 @@ __device__ __forceinline__ void stage_strip_table(float2* tab, __amdgpu_buffer_r
 {
     const int rows = W + 2 * kStripPad;
     const unsigned vo = 2 * lane < W ? (unsigned)lane * 16u : 0x80000000u;
-    for (int r = wave; r < rows; r += WAVES)
+    // staged rows -8 .. rows+7: the table's rows, and eight zero rows on
+    // either side (an out-of-range offset: the copy writes zeros)
+    for (int r = wave - 8; r < rows + 8; r += WAVES)
     {
-        const unsigned so = (unsigned)((tb + (r - kStripPad) * W) * 8);
+        const bool in = r >= 0 && r < rows;
+        const unsigned so = in ? (unsigned)((tb + (r - kStripPad) * W) * 8) : 0u;
         __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void_t*)(tab + kStripGuard + r * kStripS),
-                16, vo, so, 0, 0);
+                16, in ? vo : 0x80000000u, so, 0, 0);
     }
-    // column W of this wave's rows (lane k: row wave + k WAVES); the
-    // store waits for the loads above (they write the same LDS array)
-    const int r = wave + lane * WAVES;
-    if (r < rows) tab[kStripGuard + r * kStripS + W] = make_float2(0.f, 0.f);
+    // column W of the rows this wave copied (lane k: row wave-8 + k WAVES),
+    // after its copies have landed in LDS (they write that column too).  The
+    // wait is explicit: the compiler does not reliably order an LDS store
+    // after direct-to-LDS loads of the same array.
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    const int r = wave - 8 + lane * WAVES;
+    if (r >= 0 && r < rows) tab[kStripGuard + r * kStripS + W] = make_float2(0.f, 0.f);
 }
 
 // Pull a table's block (pad rows included) towards the compute units ahead
@@ -179,8 +195,9 @@ __device__ __forceinline__ void mac_strip(float& re, float& im, const float2 c, 
 }
 
 // Chunk loop over LDS taps, two chunks (16 rows, four slots) per block:
-// the four slots are read together, then each touched chunk of the pair is
-// multiplied.  t0 / t1: this lane's taps of chunk 0 / chunk 8.
+// the four slots of a pair that holds a touched chunk are read together,
+// then all four multiplied (rows outside the footprint are zero rows).
+// t0 / t1: this lane's taps of chunk 0 / chunk 8.
 #define GG_STRIP_PAIRS(cm)                                                     \
     {                                                                         \
         _Pragma("unroll")                                                     \
@@ -192,12 +209,10 @@ __device__ __forceinline__ void mac_strip(float& re, float& im, const float2 c, 
                                           : t1 + (pq - 4) * 16 * kStripS;     \
                 const float2 c0 = pp[0], c1 = pp[4 * kStripS];                \
                 const float2 c2 = pp[8 * kStripS], c3 = pp[12 * kStripS];     \
-                if ((cm) & (1 << (2 * pq)))                                   \
                 {                                                             \
                     mac_strip(re[4 * pq], im[4 * pq], c0, a);                 \
                     mac_strip(re[4 * pq + 1], im[4 * pq + 1], c1, a);         \
                 }                                                             \
-                if ((cm) & (2 << (2 * pq)))                                   \
                 {                                                             \
                     mac_strip(re[4 * pq + 2], im[4 * pq + 2], c2, a);         \
                     mac_strip(re[4 * pq + 3], im[4 * pq + 3], c3, a);         \
@@ -222,12 +237,10 @@ __device__ __forceinline__ void mac_strip(float& re, float& im, const float2 c, 
                                           : t1 + (7 - pq) * 16 * kStripS;     \
                 const float2 c0 = pp[0], c1 = pp[4 * kStripS];                \
                 const float2 c2 = pp[8 * kStripS], c3 = pp[12 * kStripS];     \
-                if ((cm) & (1 << (2 * pq)))                                   \
                 {                                                             \
                     mac_strip(re[4 * pq], im[4 * pq], c3, a);                 \
                     mac_strip(re[4 * pq + 1], im[4 * pq + 1], c2, a);         \
                 }                                                             \
-                if ((cm) & (2 << (2 * pq)))                                   \
                 {                                                             \
                     mac_strip(re[4 * pq + 2], im[4 * pq + 2], c1, a);         \
                     mac_strip(re[4 * pq + 3], im[4 * pq + 3], c0, a);         \
