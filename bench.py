@@ -335,11 +335,25 @@ def bench_differentiable(case, d, reps=3):
             fw.append(e0.elapsed_time(e1))
             bw.append(e1.elapsed_time(e2))
     N = case["num_points"]
+    # the degridding call alone (selection, grid-tile ordering, k_degrid)
+    y = ops.degrid(ks, d["uu"], d["vv"], d["ww"], d["weight"], case["cell_size_rad"],
+                   case["w_scale"], G, target, apply_weight=False)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        ops.degrid(ks, d["uu"], d["vv"], d["ww"], d["weight"], case["cell_size_rad"],
+                   case["w_scale"], G, target, apply_weight=False, out=y)
+    e1.record()
+    torch.cuda.synchronize()
+    dg_ms = e0.elapsed_time(e1) / 5
     out = dict(forward_ms=round(float(np.mean(fw)), 3), backward_ms=round(float(np.mean(bw)), 3),
+               degrid_call_ms=round(dg_ms, 3),
                value=round(N / ((np.mean(fw) + np.mean(bw)) * 1e-3) / 1e6, 3),
                unit="Mvis/s (forward + backward)",
                note="torch_ops.GridWProj: forward = gridder plus the loss; backward = "
-                    "degridding kernel for d/dvis and d/dweight")
+                    "degridding kernel for d/dvis and d/dweight; degrid_call_ms = one "
+                    "mi355_degrid_wproj_f_dev call (selection, grid-tile order, k_degrid)")
     log("differentiable %s" % json.dumps(out))
     return out
 

@@ -12,7 +12,7 @@
 //   k_items / scan / k_item_map   split tiles into work items
 //   k_grid_tiles     one workgroup per work item, tile in registers (or LDS)
 //   k_merge          ordered FP64 sum of partial tiles into the grid
-//   k_degrid         adjoint: one wave per visibility
+//   k_degrid         adjoint: one wave per visibility, taps dealt to all lanes
 #pragma once
 
 #include "gg_device.hpp"
@@ -1192,42 +1192,58 @@ void k_grid_nonfinite(const typename V2<T>::t* __restrict__ tables,
 // ===========================================================================
 // Degridding (adjoint)
 // ===========================================================================
-// One wave per visibility; lanes sweep the footprint's columns 64 at a time,
-// rows in order; per-lane partials are reduced across the wave at the end.
+// One wave per visibility.  The footprint's W*W taps are dealt to the 64
+// lanes in row-major order (lane l takes taps l, l+64, ...: row t / W,
+// column t % W, kept up incrementally), so every lane works whatever W is
+// and a wave's 64 table reads are one contiguous run; per-lane partials are
+// reduced across the wave at the end.  The visibilities go in the order
+// `order` gives (grid tiles of their centre cells: waves that run together
+// read neighbouring cells), or in index order when it is null.  Output per
+// visibility: sum over taps of conj-folded tap * grid cell, times the
+// weight when apply_weight (the adjoint of oskar_grid_wproj.c:190-209).
 template <typename T>
 __global__ __launch_bounds__(256)
 void k_degrid(const typename V2<T>::t* __restrict__ tables,
-        const int4* __restrict__ meta, const T* __restrict__ weight,
-        unsigned long long n, const typename V2<T>::t* __restrict__ grid,
-        int grid_size, int apply_weight, typename V2<T>::t* __restrict__ out)
+        const int4* __restrict__ meta, const unsigned* __restrict__ order,
+        const T* __restrict__ weight, unsigned long long n,
+        const typename V2<T>::t* __restrict__ grid, int grid_size, int apply_weight,
+        typename V2<T>::t* __restrict__ out)
 {
     typedef typename V2<T>::t T2;
     const int lane = lane_id();
     const unsigned long long nw = (unsigned long long)gridDim.x * (blockDim.x >> 6);
     const unsigned long long w0 = (unsigned long long)blockIdx.x * (blockDim.x >> 6) +
             (unsigned)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    for (unsigned long long i = w0; i < n; i += nw)
+    for (unsigned long long q = w0; q < n; q += nw)
     {
+        const unsigned long long i = order ? (unsigned long long)ld_const(order, q) : q;
         const int4 mt = ld_const(meta, i);
         T re = (T)0, im = (T)0;
         if (!(mt.z & kFlagSkip))
         {
             const int s = meta_support(mt.z);
             const int W = 2 * s + 1;
+            const int n2 = W * W;
             const bool neg_u = mt.z & kFlagNegU, neg_v = mt.z & kFlagNegV;
             const T cj = (mt.z & kFlagConj) ? (T)-1 : (T)1;
-            for (int jr = -s; jr <= s; ++jr)
+            // 64 taps further on: dr rows and dc columns (one more row when
+            // the column passes W)
+            const int dr = 64 / W, dc = 64 - dr * W;
+            int r = lane / W, c = lane - (lane / W) * W;
+            const T2* tab = tables + (unsigned)mt.w;                 // table row 0
+            const T2* g0 = grid + (size_t)(mt.y - s) * grid_size + (mt.x - s);
+            for (int t = lane; t < n2; t += 64)
             {
-                const T2* krow = tables + (unsigned)mt.w + (neg_v ? s - jr : s + jr) * W;
-                const T2* grow = grid + (size_t)(mt.y + jr) * grid_size + mt.x;
-                for (int k = -s + lane; k <= s; k += 64)
-                {
-                    const T2 c = krow[neg_u ? s - k : s + k];
-                    const T2 g = grow[k];
-                    const T ci = c.y * cj;
-                    re = fma(c.x, g.x, re); re = fma(ci, g.y, re);
-                    im = fma(c.x, g.y, im); im = fma(-ci, g.x, im);
-                }
+                const int tr = neg_v ? W - 1 - r : r;
+                const int tc = neg_u ? W - 1 - c : c;
+                const T2 cv = tab[tr * W + tc];
+                const T2 g = g0[r * grid_size + c];
+                const T ci = cv.y * cj;
+                re = fma(cv.x, g.x, re); re = fma(ci, g.y, re);
+                im = fma(cv.x, g.y, im); im = fma(-ci, g.x, im);
+                r += dr;
+                c += dc;
+                if (c >= W) { c -= W; ++r; }
             }
         }
         re = wave_sum(re);
@@ -1238,6 +1254,21 @@ void k_degrid(const typename V2<T>::t* __restrict__ tables,
             T2 v; v.x = re * w; v.y = im * w;
             out[i] = v;
         }
+    }
+}
+
+// Degridding order keys: the 64 x 64 grid tile of each visibility's centre
+// cell (tiles_x per tile row), skipped visibilities under skip_key.
+__global__ void k_degrid_keys(const int4* __restrict__ meta, unsigned long long n, int tiles_x,
+        unsigned skip_key, unsigned* __restrict__ keys, unsigned* __restrict__ vals)
+{
+    for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
+            i < n; i += (unsigned long long)gridDim.x * blockDim.x)
+    {
+        const int4 m = meta[i];
+        keys[i] = (m.z & kFlagSkip) ? skip_key
+                                     : (unsigned)((m.y >> 6) * tiles_x + (m.x >> 6));
+        vals[i] = (unsigned)i;
     }
 }
 

@@ -58,6 +58,8 @@ int ok()
 // (on the host it is a stand-in, not a device address); init_ctx resolves
 // each device's address once with hipGetSymbolAddress and the pipeline
 // carves its buffers out of that address.
+// Degridding calls of at least this many visibilities go in grid-tile order.
+constexpr size_t kDegridSortMin = 4096;
 constexpr size_t kNoAllocBytes = (size_t)64 << 20;
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
@@ -1410,11 +1412,38 @@ int degrid_dev(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     hipLaunchKernelGGL((k_select<T, T>), dim3(grid_blocks(chunks, 1, 65535)), dim3(kSelThreads), 0,
             C->stream, sa, (unsigned)chunks);
     HIP_TRY(hipGetLastError());
+    // Order by the 64 x 64 grid tile of the centre cell (a stable radix
+    // sort of N keys), so that the waves running together read neighbouring
+    // grid cells; small calls keep index order.
+    const unsigned* order = nullptr;
+    if (N >= kDegridSortMin)
+    {
+        const int tiles_x = (G + 63) / 64;
+        const unsigned long long ntile = (unsigned long long)tiles_x * tiles_x;
+        TRY(ensure(C, C->scalars, SC_COUNT * sizeof(unsigned long long)));
+        unsigned long long* sc = C->scalars.as<unsigned long long>();
+        for (int b = 0; b < 2; ++b)
+        {
+            TRY(ensure(C, C->keys[b], N * sizeof(unsigned)));
+            TRY(ensure(C, C->vals[b], N * sizeof(unsigned)));
+        }
+        // the sort's count: N in sc[SC_VISITS] (two 32-bit words)
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, C->stream,
+                reinterpret_cast<unsigned*>(sc + SC_VISITS), 1ull, (unsigned)N);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, C->stream,
+                reinterpret_cast<unsigned*>(sc + SC_VISITS) + 1, 1ull, 0u);
+        hipLaunchKernelGGL(k_degrid_keys, dim3(grid_blocks(N, 256, 16384)), dim3(256), 0, C->stream,
+                C->meta.as<int4>(), (unsigned long long)N, tiles_x, (unsigned)ntile,
+                C->keys[0].as<unsigned>(), C->vals[0].as<unsigned>());
+        HIP_TRY(hipGetLastError());
+        int cur = 0;
+        TRY(radix_sort(C, N, sc + SC_VISITS, bits_for(ntile + 1), &cur));
+        order = C->vals[cur].as<unsigned>();
+    }
     hipLaunchKernelGGL(k_degrid<T>, dim3(grid_blocks(N, 4, 65535)), dim3(256), 0, C->stream,
-            C->tables.as<T2>(), C->meta.as<int4>(), weight, (unsigned long long)N,
+            C->tables.as<T2>(), C->meta.as<int4>(), order, weight, (unsigned long long)N,
             reinterpret_cast<const T2*>(grid), G, apply_weight, reinterpret_cast<T2*>(out));
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(C->stream));
     return ok();
 }
 
