@@ -189,6 +189,21 @@ def bench_batched(a):
         off += n
     args = (ks, jobs, cat["uu"], cat["vv"], cat["ww"], cat["vis"], cat["weight"])
     ctx = ops.context()
+    if a.batched_leg != "all":
+        # one leg alone, default options (for kernel traces of that leg)
+        fn = ((lambda: ops.grid_batched_graph(*args, ctx=ctx)) if a.batched_leg == "graph"
+              else (lambda: ops.grid_batched(*args, ctx=ctx)))
+        for _ in range(max(a.warmup, 1)):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(a.steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        log("%s batched pass: %.3f ms" % (a.batched_leg, e0.elapsed_time(e1) / a.steps))
+        return
     best = ops.autotune_batched(*args, ctx=ctx)
     log("autotuned: %.3f ms per pass" % best)
 
@@ -233,6 +248,11 @@ def bench_batched(a):
                    "autotuned": {"grid_kernel": ctx.get_option(_lib.OPT_GRID_KERNEL),
                                  "visits_per_item": ctx.get_option(_lib.OPT_VISITS_PER_ITEM),
                                  "min_stage_run": ctx.get_option(_lib.OPT_MIN_STAGE_RUN),
+                                 "min_stage_run_per_class_w33_w65_w97_w111": [
+                                     ctx.get_option(k) for k in (
+                                         _lib.OPT_MIN_STAGE_RUN_W33, _lib.OPT_MIN_STAGE_RUN_W65,
+                                         _lib.OPT_MIN_STAGE_RUN_W97,
+                                         _lib.OPT_MIN_STAGE_RUN_W111)],
                                  "item_group": ctx.get_option(_lib.OPT_ITEM_GROUP)}},
         "batched": {"graph_ms": round(graph_ms, 3), "eager_batched_ms": round(eager_ms, 3),
                     "sequential_64_ms": round(seq_ms, 3),
@@ -478,6 +498,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--batched-leg", choices=("all", "eager", "graph"), default="all",
+                    help="config 3: time only the eager batched pass or only the graph "
+                         "replays, with default options (no autotuning, no other legs)")
     ap.add_argument("--compensated", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")

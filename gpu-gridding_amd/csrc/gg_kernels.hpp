@@ -736,15 +736,17 @@ __global__ void k_run_starts(const unsigned* __restrict__ flag,
 
 // Records of the sorted visits, in visit order (one load per batch of
 // visits in the strip kernel instead of an index load and a dependent one),
-// with kVisitStaged set where the visit's run is staged in LDS (at least
-// min_run visits and a table at most max_w wide) and kVisitRunStart where
+// with kVisitStaged set where the visit's run is staged in LDS (a table at
+// most max_w wide, and at least the threshold of its support class of
+// visits: min_run.x .. .w for tables at most 33, 65, 97 and max_w wide) and
+// kVisitRunStart where
 // the visit starts a run; y holds the visit's placement in its region (see
 // below) and vlds the visit's part of its staged LDS offsets.
 template <typename T>
 __global__ void k_gather_visits(const unsigned* __restrict__ vals,
         const int4* __restrict__ meta, const typename V4<T>::t* __restrict__ amp,
         const unsigned* __restrict__ run_flag, const unsigned* __restrict__ run_excl,
-        const unsigned* __restrict__ run_start, unsigned min_run, int max_w, int lds_stride,
+        const unsigned* __restrict__ run_start, uint4 min_run, int max_w, int lds_stride,
         const unsigned* __restrict__ keys, int key_table_bits,
         const JobDev<T>* __restrict__ jobs, const unsigned* __restrict__ job_tile_base,
         int num_jobs, int region_h,
@@ -758,8 +760,10 @@ __global__ void k_gather_visits(const unsigned* __restrict__ vals,
         const unsigned i = vals[p];
         int4 m = meta[i];
         const unsigned r = run_excl[p] + run_flag[p] - 1u;
-        if (run_start[r + 1] - run_start[r] >= min_run && 2 * meta_support(m.z) + 1 <= max_w)
-            m.z |= kVisitStaged;
+        const int W = 2 * meta_support(m.z) + 1;
+        const unsigned mr = W <= 33 ? min_run.x : W <= 65 ? min_run.y : W <= 97 ? min_run.z
+                                                                           : min_run.w;
+        if (run_start[r + 1] - run_start[r] >= mr && W <= max_w) m.z |= kVisitStaged;
         if (run_flag[p]) m.z |= kVisitRunStart;
         const typename V4<T>::t am = amp[i];
         vamp[p] = {am.x, am.z};            // the weighted amplitude (re, im)
@@ -1147,16 +1151,23 @@ void k_merge(const unsigned* __restrict__ item_off, const unsigned* __restrict__
 // has finished.  The cells it touches end up NaN or +-Inf whatever the
 // order of the additions, exactly as the serial code leaves them; keeping
 // these few visibilities out of the tiled kernels lets those read zero
-// rows for out-of-footprint taps (0 * Inf would be NaN).
+// rows for out-of-footprint taps (0 * Inf would be NaN).  A pass whose jobs
+// counted no such visibility (job_out[3j+2], from k_job_reduce) returns at
+// once: the host-sync-free pipeline launches this kernel without knowing.
 template <typename T>
 __global__ __launch_bounds__(256)
 void k_grid_nonfinite(const typename V2<T>::t* __restrict__ tables,
         const int4* __restrict__ meta, const typename V4<T>::t* __restrict__ amp,
         const JobDev<T>* __restrict__ jobs, const unsigned long long* __restrict__ job_vis_off,
-        int num_jobs, unsigned long long n)
+        int num_jobs, unsigned long long n, const unsigned long long* __restrict__ job_out)
 {
     typedef typename V2<T>::t T2;
     const int lane = lane_id();
+    {
+        unsigned long long cnt = 0;
+        for (int j = lane; j < num_jobs; j += 64) cnt += job_out[3 * j + 2];
+        if (!__ballot(cnt != 0)) return;
+    }
     const unsigned long long nw = (unsigned long long)gridDim.x * (blockDim.x >> 6);
     const unsigned long long w0 = (unsigned long long)blockIdx.x * (blockDim.x >> 6) +
             (unsigned)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));

@@ -180,6 +180,14 @@ struct mi355_ctx
     bool cache_kernels = false;
     int grid_kernel = 0;            // 0 auto (strips for plain FP32), 1 direct 64x64, 2 staged 64x64
     unsigned min_stage_run = 3;
+    // per support class (tables at most 33, 65, 97, 111 wide): the strip
+    // kernel's staged-run threshold, 0 = min_stage_run
+    unsigned msr_class[4] = {0, 0, 0, 0};
+    // the autotuner's choices per workload shape (signature -> settings)
+    struct Tuned { std::vector<unsigned char> sig; int kernel; unsigned vpi, msr, group;
+                   unsigned cls[4]; double ms; };
+    std::vector<Tuned> tuned;
+    bool tune_cache = true;
     unsigned item_group = 1;        // strip kernel: consecutive work items per XCD
     int num_cu = 256;
 
@@ -270,6 +278,15 @@ namespace {
 
 std::mutex g_default_lock;
 mi355_ctx* g_default[64] = {};
+
+// The strip kernel's staged-run thresholds per support class (tables at
+// most 33, 65, 97 and 111 wide).
+uint4 stage_thresholds(const mi355_ctx* C)
+{
+    const unsigned* c = C->msr_class;
+    const unsigned b = C->min_stage_run;
+    return make_uint4(c[0] ? c[0] : b, c[1] ? c[1] : b, c[2] ? c[2] : b, c[3] ? c[3] : b);
+}
 
 // Work this context has queued that may still read its buffers: everything
 // on its current stream, and on the stream it used before the last
@@ -645,7 +662,7 @@ int grid_nonfinite(mi355_ctx* C, unsigned long long count, int num_jobs,
     hipLaunchKernelGGL(k_grid_nonfinite<T>, dim3(grid_blocks(n_total, 4, 65535)), dim3(256), 0,
             C->stream, C->tables.as<T2>(), C->meta.as<int4>(), C->amp.as<T4>(),
             C->jobs_d.as<JobDev<T>>(), C->job_vis_off_d.as<unsigned long long>(), num_jobs,
-            n_total);
+            n_total, C->job_out_d.as<unsigned long long>());
     HIP_TRY(hipGetLastError());
     return MI355_OK;
 }
@@ -928,7 +945,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
         TRY(ensure(C, C->vlds, T_vis * sizeof(int)));
         hipLaunchKernelGGL(k_gather_visits<T>, dim3(grid_blocks(T_vis, 256, 65535)), dim3(256), 0,
                 C->stream, svals, C->meta.as<int4>(), C->amp.as<T4>(), run_flag, run_excl,
-                C->run_start.as<unsigned>(), C->min_stage_run, kStripMaxW, kStripS, skeys,
+                C->run_start.as<unsigned>(), stage_thresholds(C), kStripMaxW, kStripS, skeys,
                 key_table_bits, C->jobs_d.as<JobDev<T>>(), C->job_tile_base_d.as<unsigned>(),
                 num_jobs, th, sc + SC_VISITS, C->vmeta.as<int4>(), C->vamp.as<T2>(),
                 C->vlds.as<int>());
@@ -1903,6 +1920,15 @@ MI355_EXPORT int mi355_ctx_set_option(mi355_ctx* C, int key, long long value)
         if (value < 1 || value > 4096) return fail(MI355_ERR_INVALID_ARG, "item group %lld", value);
         C->item_group = (unsigned)value;
         break;
+    case MI355_OPT_MIN_STAGE_RUN_W33:
+    case MI355_OPT_MIN_STAGE_RUN_W65:
+    case MI355_OPT_MIN_STAGE_RUN_W97:
+    case MI355_OPT_MIN_STAGE_RUN_W111:
+        if (value < 0) return fail(MI355_ERR_INVALID_ARG, "min stage run %lld", value);
+        C->msr_class[key - MI355_OPT_MIN_STAGE_RUN_W33] =
+                (unsigned)std::min<long long>(value, 0xFFFFFFFFll);
+        break;
+    case MI355_OPT_AUTOTUNE_CACHE: C->tune_cache = value != 0; break;
     case MI355_OPT_MAX_WORKSPACE:
         if (value < 0) return fail(MI355_ERR_INVALID_ARG, "workspace cap %lld", value);
         C->max_workspace = (unsigned long long)value;
@@ -1924,6 +1950,13 @@ MI355_EXPORT int mi355_ctx_get_option(const mi355_ctx* C, int key, long long* va
     case MI355_OPT_GRID_KERNEL: *value = C->grid_kernel; break;
     case MI355_OPT_MIN_STAGE_RUN: *value = C->min_stage_run; break;
     case MI355_OPT_ITEM_GROUP: *value = C->item_group; break;
+    case MI355_OPT_MIN_STAGE_RUN_W33:
+    case MI355_OPT_MIN_STAGE_RUN_W65:
+    case MI355_OPT_MIN_STAGE_RUN_W97:
+    case MI355_OPT_MIN_STAGE_RUN_W111:
+        *value = C->msr_class[key - MI355_OPT_MIN_STAGE_RUN_W33];
+        break;
+    case MI355_OPT_AUTOTUNE_CACHE: *value = C->tune_cache; break;
     case MI355_OPT_MAX_WORKSPACE: *value = (long long)C->max_workspace; break;
     default: return fail(MI355_ERR_INVALID_ARG, "unknown option %d", key);
     }
@@ -2036,6 +2069,7 @@ MI355_EXPORT int mi355_grid_wproj_f_dev_noalloc(mi355_ctx* C, size_t P, const in
     Z->compensated = C->compensated;
     Z->grid_kernel = C->grid_kernel;
     Z->min_stage_run = C->min_stage_run;
+    std::copy(C->msr_class, C->msr_class + 4, Z->msr_class);
     Z->item_group = C->item_group;
     Z->timing = C->timing;
     Z->cache_kernels = false;          // (the scratch is shared: tables are rebuilt per call)
@@ -2147,6 +2181,7 @@ MI355_EXPORT int mi355_grid_wproj_f_batched_graph_dev(mi355_ctx* C, size_t P,
     C->compensated = parent->compensated;
     C->grid_kernel = parent->grid_kernel;
     C->min_stage_run = parent->min_stage_run;
+    std::copy(parent->msr_class, parent->msr_class + 4, C->msr_class);
     C->item_group = parent->item_group;
     C->max_workspace = parent->max_workspace;
     C->timing = parent->timing;
@@ -2161,9 +2196,11 @@ MI355_EXPORT int mi355_grid_wproj_f_batched_graph_dev(mi355_ctx* C, size_t P,
         key.insert(key.end(), b, b + n);
     };
     const void* ptrs[6] = {conv, uu, vv, ww, vis, weight};
-    const long long opts[7] = {(long long)C->visits_per_item, (long long)C->min_stage_run,
-                               C->grid_kernel, C->compensated, (long long)C->item_group,
-                               (long long)C->max_workspace, (long long)(intptr_t)C->stream};
+    const long long opts[11] = {(long long)C->visits_per_item, (long long)C->min_stage_run,
+                                C->grid_kernel, C->compensated, (long long)C->item_group,
+                                (long long)C->max_workspace, (long long)(intptr_t)C->stream,
+                                C->msr_class[0], C->msr_class[1], C->msr_class[2],
+                                C->msr_class[3]};
     put(&P, sizeof(P));
     put(support, P * sizeof(int));
     put(&ov, sizeof(ov));
@@ -2252,59 +2289,135 @@ MI355_EXPORT int mi355_autotune_f_batched_dev(mi355_ctx* C, size_t P, const int*
     std::vector<JobHost<float, float>> J;
     TRY(batched_jobs(P, support, ov, csh, conv, num_jobs, jobs, uu, vv, ww, vis, weight, J));
     HIP_TRY(hipSetDevice(C->device));
-    // Trial outputs: one scratch grid (the largest), norm and count per
-    // job, so the caller's grids are not touched.
-    int gmax = 0;
-    for (int j = 0; j < num_jobs; ++j) gmax = std::max(gmax, J[j].grid_size);
-    DevBuf tgrid, tnorm, tskip;
-    TRY(ensure(C, tgrid, (size_t)gmax * gmax * sizeof(float2)));
-    TRY(ensure(C, tnorm, (size_t)num_jobs * sizeof(double)));
-    TRY(ensure(C, tskip, (size_t)num_jobs * sizeof(unsigned long long)));
+    // The workload's shape: what the best settings depend on (not the
+    // pointers or the values).
+    std::vector<unsigned char> sig;
+    auto put = [&](const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        sig.insert(sig.end(), b, b + n);
+    };
+    put(&P, sizeof(P));
+    put(support, P * sizeof(int));
+    put(&ov, sizeof(ov));
+    put(&csh, sizeof(csh));
+    put(&num_jobs, sizeof(num_jobs));
     for (int j = 0; j < num_jobs; ++j)
     {
-        J[j].grid = tgrid.as<float2>();         // (results are discarded)
+        put(&J[j].grid_size, sizeof(int));
+        put(&J[j].num_points, sizeof(J[j].num_points));
+    }
+    auto apply = [&](int kernel, unsigned vpi, unsigned msr, unsigned group, const unsigned* cls) {
+        C->grid_kernel = kernel;
+        C->visits_per_item = vpi;
+        C->min_stage_run = msr;
+        C->item_group = group;
+        std::copy(cls, cls + 4, C->msr_class);
+    };
+    if (C->tune_cache)
+        for (const auto& t : C->tuned)
+            if (t.sig == sig)
+            {
+                apply(t.kernel, t.vpi, t.msr, t.group, t.cls);
+                if (best_ms) *best_ms = t.ms;
+                return ok();
+            }
+    // Trial outputs: a private scratch grid per job (sized for it), norm and
+    // count per job, so the caller's outputs are not touched and the trial
+    // jobs write as many grids as the real call does.
+    size_t cells = 0;
+    for (int j = 0; j < num_jobs; ++j) cells += (size_t)J[j].grid_size * J[j].grid_size;
+    DevBuf tgrid, tnorm, tskip;
+    TRY(ensure(C, tgrid, cells * sizeof(float2)));
+    TRY(ensure(C, tnorm, (size_t)num_jobs * sizeof(double)));
+    TRY(ensure(C, tskip, (size_t)num_jobs * sizeof(unsigned long long)));
+    for (int j = 0, off = 0; j < num_jobs; ++j)
+    {
+        J[j].grid = tgrid.as<float2>() + off;      // (results are discarded)
         J[j].norm = tnorm.as<double>() + j;
         J[j].skipped = tskip.as<unsigned long long>() + j;
+        off += J[j].grid_size * J[j].grid_size;
     }
     const bool timing0 = C->timing;
     const unsigned vpi0 = C->visits_per_item, msr0 = C->min_stage_run, ig0 = C->item_group;
     const int gk0 = C->grid_kernel;
+    unsigned cls0[4];
+    std::copy(C->msr_class, C->msr_class + 4, cls0);
     C->timing = true;
-    struct Cand { int kernel; unsigned vpi, msr, group; };
-    std::vector<Cand> cands;
+    int st = MI355_OK;
+    // one timed pass with the context's current settings
+    auto trial = [&](double& ms) -> int {
+        const int rs = run_pipeline<float, float>(C, P, support, ov, csh,
+                reinterpret_cast<const float2*>(conv), num_jobs, J.data(), uu, vv, ww, vis,
+                weight, nullptr);
+        if (rs != MI355_OK) return rs;
+        collect_timing(C);
+        ms = C->stats.ms_total;
+        return MI355_OK;
+    };
+    struct Setting { int kernel; unsigned vpi, msr, group; unsigned cls[4]; };
+    Setting best{gk0, vpi0, msr0, ig0, {cls0[0], cls0[1], cls0[2], cls0[3]}};
+    double best_t = 1e300, t = 0;
+    // warm-up (buffers grow to their sizes, tables are expanded)
+    apply(best.kernel, best.vpi, best.msr, best.group, best.cls);
+    st = trial(t);
+    // Stage 1: kernel / geometry, work-item size, item grouping, with one
+    // staged-run threshold for every class.
+    const unsigned zero4[4] = {0, 0, 0, 0};
     for (int kern : {0, 2})
         for (unsigned vpi : {1024u, 2048u, 4096u, 8192u})
-            for (unsigned msr : {2u, 3u, 4u, 6u})
-                for (unsigned grp : {1u, 8u})
-                    if (kern == 0 || (msr == 4u && grp == ig0)) cands.push_back({kern, vpi, msr, grp});
-    double best = 1e300;
-    Cand bc = {gk0, vpi0, msr0, ig0};
-    int st = MI355_OK;
-    for (const Cand& c : cands)
+            for (unsigned grp : {1u, 8u})
+            {
+                if (st != MI355_OK) break;
+                if (kern == 2 && grp != 1u) continue;
+                apply(kern, vpi, kern == 2 ? 4u : 3u, grp, zero4);
+                st = trial(t);
+                if (st == MI355_OK && t < best_t)
+                {
+                    best_t = t;
+                    best = Setting{kern, vpi, C->min_stage_run, grp, {0, 0, 0, 0}};
+                }
+            }
+    // Stage 2 (strip kernel): each support class's threshold in turn, the
+    // others held at their best so far; two passes per value, the faster
+    // counts.
+    if (st == MI355_OK && best.kernel == 0)
     {
-        C->grid_kernel = c.kernel;
-        C->visits_per_item = c.vpi;
-        C->min_stage_run = c.msr;
-        C->item_group = c.group;
-        double t = 0;
-        for (int rep = 0; rep < 2 && st == MI355_OK; ++rep)   // first run warms up
+        for (int c = 0; c < 4 && st == MI355_OK; ++c)
         {
-            st = run_pipeline<float, float>(C, P, support, ov, csh,
-                    reinterpret_cast<const float2*>(conv), num_jobs, J.data(), uu, vv, ww, vis,
-                    weight, nullptr);
-            if (st == MI355_OK) { collect_timing(C); t = C->stats.ms_total; }
+            unsigned cls[4];
+            std::copy(best.cls, best.cls + 4, cls);
+            for (unsigned v : {1u, 2u, 3u, 4u, 6u, 8u, 12u})
+            {
+                cls[c] = v;
+                apply(best.kernel, best.vpi, best.msr, best.group, cls);
+                double t2 = 0;
+                st = trial(t);
+                if (st == MI355_OK) st = trial(t2);
+                if (st != MI355_OK) break;
+                t = std::min(t, t2);
+                if (t < best_t)
+                {
+                    best_t = t;
+                    std::copy(cls, cls + 4, best.cls);
+                }
+            }
         }
-        if (st != MI355_OK) break;
-        if (t < best) { best = t; bc = c; }
     }
     C->timing = timing0;
-    C->grid_kernel = st == MI355_OK ? bc.kernel : gk0;
-    C->visits_per_item = st == MI355_OK ? bc.vpi : vpi0;
-    C->min_stage_run = st == MI355_OK ? bc.msr : msr0;
-    C->item_group = st == MI355_OK ? bc.group : ig0;
     release(C, tgrid); release(C, tnorm); release(C, tskip);
-    if (st != MI355_OK) return st;
-    if (best_ms) *best_ms = best;
+    if (st != MI355_OK)
+    {
+        apply(gk0, vpi0, msr0, ig0, cls0);
+        return st;
+    }
+    apply(best.kernel, best.vpi, best.msr, best.group, best.cls);
+    mi355_ctx::Tuned rec{sig, best.kernel, best.vpi, best.msr, best.group,
+              {best.cls[0], best.cls[1], best.cls[2], best.cls[3]}, best_t};
+    bool stored = false;
+    for (auto& t0 : C->tuned)
+        if (t0.sig == sig) { t0 = rec; stored = true; }
+    if (!stored) C->tuned.push_back(rec);
+    if (best_ms) *best_ms = best_t;
     return ok();
 }
 

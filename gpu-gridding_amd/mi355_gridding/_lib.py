@@ -33,6 +33,11 @@ OPT_PHASE_TIMING = 3
 OPT_CACHE_KERNELS = 4
 OPT_GRID_KERNEL = 5
 OPT_MIN_STAGE_RUN = 6
+OPT_MIN_STAGE_RUN_W33 = 9
+OPT_MIN_STAGE_RUN_W65 = 10
+OPT_MIN_STAGE_RUN_W97 = 11
+OPT_MIN_STAGE_RUN_W111 = 12
+OPT_AUTOTUNE_CACHE = 13
 OPT_ITEM_GROUP = 7
 OPT_MAX_WORKSPACE = 8
 

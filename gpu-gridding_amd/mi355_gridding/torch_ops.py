@@ -271,9 +271,11 @@ def grid_batched_graph(ks, jobs, uu, vv, ww, vis, weight, ctx=None):
 
 
 def autotune_batched(ks, jobs, uu, vv, ww, vis, weight, ctx=None):
-    """Pick the gridding kernel, work-item size and shortest staged run for
-    this workload (mi355_autotune_f_batched_dev); the context keeps them.
-    Returns the best time per pass in ms."""
+    """Pick the gridding kernel, work-item size, item grouping and the
+    shortest staged run per support class for this workload
+    (mi355_autotune_f_batched_dev); the context keeps them, and caches the
+    choice per workload shape.  Returns the chosen settings' time per pass
+    in ms."""
     best = ctypes.c_double(0.0)
     _batched_call(_lib.load().mi355_autotune_f_batched_dev, ks, jobs, uu, vv, ww, vis, weight,
                   ctx, ctypes.byref(best))

@@ -113,6 +113,17 @@ MI355_EXPORT int mi355_ctx_get_stream(mi355_ctx* ctx, void** hip_stream);
                                         context may hold (0: no cap, the default);
                                         a call that would need more fails with
                                         MI355_ERR_OUT_OF_MEMORY before allocating */
+/* Strip kernel (FP32): shortest same-table run staged in LDS per support
+ * class, tables at most 33, 65, 97 and 111 wide (0, the default: the value
+ * of MI355_OPT_MIN_STAGE_RUN). */
+#define MI355_OPT_MIN_STAGE_RUN_W33  9
+#define MI355_OPT_MIN_STAGE_RUN_W65  10
+#define MI355_OPT_MIN_STAGE_RUN_W97  11
+#define MI355_OPT_MIN_STAGE_RUN_W111 12
+#define MI355_OPT_AUTOTUNE_CACHE  13 /* 1 (default): the autotuner answers a
+                                        workload shape it has tuned on this
+                                        context from its cache; 0: it always
+                                        runs its trials (and updates the cache) */
 MI355_EXPORT int mi355_ctx_set_option(mi355_ctx* ctx, int key, long long value);
 MI355_EXPORT int mi355_ctx_get_option(const mi355_ctx* ctx, int key, long long* value);
 
@@ -285,13 +296,21 @@ MI355_EXPORT int mi355_grid_wproj_f_batched_graph_dev(mi355_ctx* ctx,
         const mi355_job_f* jobs, const float* uu, const float* vv,
         const float* ww, const float* vis, const float* weight);
 
-/* Autotuner for a batched workload: times the pipeline (into scratch
- * outputs; the job table's grids are not touched) for each gridding kernel
- * / tile geometry (16-column strips on 256x128 regions, or 64x64 tiles),
- * work-item size and shortest staged run, and leaves the fastest settings
- * in the context's options (MI355_OPT_GRID_KERNEL, _VISITS_PER_ITEM,
- * _MIN_STAGE_RUN, _ITEM_GROUP).  *best_ms receives that setting's time per pass.  No
- * reference counterpart. */
+/* Autotuner for a batched workload: times the pipeline into private
+ * outputs (one scratch grid per job; the job table's grids are not
+ * touched), first over the gridding kernel / tile geometry (16-column
+ * strips on 256x128 regions, or 64x64 tiles), work-item size and item
+ * grouping, then, for the strip kernel, over the shortest staged run of
+ * each support class in turn (tables at most 33, 65, 97, 111 wide), and
+ * leaves the fastest settings in the context's options
+ * (MI355_OPT_GRID_KERNEL, _VISITS_PER_ITEM, _ITEM_GROUP, _MIN_STAGE_RUN,
+ * _MIN_STAGE_RUN_W33 .. _W111).  The choice is cached on the context per
+ * workload shape (job count, visibility counts, grid sizes, w-planes and
+ * supports, oversampling, kernel-stack size): a later call with the same
+ * shape applies it without trials (see MI355_OPT_AUTOTUNE_CACHE).
+ * *best_ms receives the chosen settings' time per pass.  Counterpart:
+ * OskarTuningParams (oskar_grid_wproj_gpu.cu:1646-1655, set at
+ * :1683-1688), fixed there. */
 MI355_EXPORT int mi355_autotune_f_batched_dev(mi355_ctx* ctx,
         size_t num_w_planes, const int* support, int oversample,
         int conv_size_half, const float* conv_func, int num_jobs,

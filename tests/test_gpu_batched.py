@@ -119,14 +119,41 @@ def _graph_replays(torch, ops, ctx, args, cat, jobs):
         assert same_bits(g0 * 2, job["grid"].cpu().numpy()) and s0 == job["num_skipped"].item()
 
 
+TUNED = (_lib.OPT_GRID_KERNEL, _lib.OPT_VISITS_PER_ITEM, _lib.OPT_MIN_STAGE_RUN,
+         _lib.OPT_ITEM_GROUP, _lib.OPT_MIN_STAGE_RUN_W33, _lib.OPT_MIN_STAGE_RUN_W65,
+         _lib.OPT_MIN_STAGE_RUN_W97, _lib.OPT_MIN_STAGE_RUN_W111)
+
+
 def test_autotune_keeps_results_and_sets_options(gpu):
+    """The trials write private grids (the caller's outputs stay untouched);
+    the choice is cached per workload shape: a second call with the same
+    shape (other values) returns it without trials, a call with the cache
+    switched off runs them again; a different shape is tuned afresh."""
     torch, ops = gpu
     ks, cat, jobs = _batched_case(torch, ops, num_jobs=3, seed=4)
     args = (ks, jobs, cat["uu"], cat["vv"], cat["ww"], cat["vis"], cat["weight"])
-    ctx = ops.context()
+    ctx = _lib.Context(torch.cuda.current_device())
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
     _zero(jobs)
+    import time
+    t0 = time.perf_counter()
     best = ops.autotune_batched(*args, ctx=ctx)
+    t_first = time.perf_counter() - t0
     assert best > 0
+    chosen = [ctx.get_option(k) for k in TUNED]
+    # scramble the options: the cached answer puts the choice back
+    ctx.set_option(_lib.OPT_VISITS_PER_ITEM, 333)
+    ctx.set_option(_lib.OPT_MIN_STAGE_RUN_W65, 7)
+    cat["vis"].mul_(3)                        # same shape, other values
+    t0 = time.perf_counter()
+    best2 = ops.autotune_batched(*args, ctx=ctx)
+    t_cached = time.perf_counter() - t0
+    assert best2 == best and [ctx.get_option(k) for k in TUNED] == chosen
+    assert t_cached < 0.25 * t_first, (t_cached, t_first)
+    ctx.set_option(_lib.OPT_AUTOTUNE_CACHE, 0)
+    best3 = ops.autotune_batched(*args, ctx=ctx)
+    assert best3 > 0 and best3 != best
+    ctx.set_option(_lib.OPT_AUTOTUNE_CACHE, 1)
     # the caller's outputs were not touched by the trials
     for job in jobs:
         assert job["grid"].abs().max().item() == 0 and job["num_skipped"].item() == -1
@@ -134,6 +161,7 @@ def test_autotune_keeps_results_and_sets_options(gpu):
     torch.cuda.synchronize()
     for job in jobs:
         assert job["num_skipped"].item() >= 0
+    ctx.close()
 
 
 def test_graph_survives_other_calls_on_the_context(gpu):
@@ -183,7 +211,8 @@ def test_graph_survives_other_calls_on_the_context(gpu):
     # options; with the options put back, the key matches the graph built
     # above, which must replay unchanged
     keys = (_lib.OPT_GRID_KERNEL, _lib.OPT_VISITS_PER_ITEM, _lib.OPT_MIN_STAGE_RUN,
-            _lib.OPT_ITEM_GROUP)
+            _lib.OPT_ITEM_GROUP, _lib.OPT_MIN_STAGE_RUN_W33, _lib.OPT_MIN_STAGE_RUN_W65,
+            _lib.OPT_MIN_STAGE_RUN_W97, _lib.OPT_MIN_STAGE_RUN_W111)
     saved = [ctx.get_option(k) for k in keys]
     ops.autotune_batched(*args, ctx=ctx)
     for k, v in zip(keys, saved):

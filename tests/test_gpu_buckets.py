@@ -10,7 +10,8 @@ regions outer and columns inner; its key is (region << bits) | table, with
 table = plane * 9 + |off_u| * 3 + |off_v| (oversample 4: |off| <= 2); the
 visits are sorted stably by key.  A run is a maximal stretch of equal keys;
 its visits are staged when it has at least min_stage_run of them and the
-table is at most 111 wide.  This replaces the per-tile count and scatter of
+table is at most 111 wide; the threshold may differ per support class
+(tables at most 33, 65, 97, 111 wide).  This replaces the per-tile count and scatter of
 oskar_grid_wproj_gpu.cu:359-649, so grid parity alone would not catch, for
 example, a visit dropped from one region and duplicated in another.
 """
@@ -27,7 +28,7 @@ pytestmark = pytest.mark.gpu
 RW, RH, MAXW = 256, 128, 111
 
 
-def restate(c, min_run):
+def restate(c, min_runs):
     sel = oracle.select_f(c)
     G = c["grid_size"]
     keep = np.flatnonzero(sel["skip"] == 0)
@@ -56,14 +57,20 @@ def restate(c, min_run):
     run_id = np.cumsum(start) - 1
     run_len = np.bincount(run_id)[run_id]
     W = 2 * sel["support"][vis] + 1
-    staged = (run_len >= min_run) & (W <= MAXW)
+    thr = np.select([W <= 33, W <= 65, W <= 97], list(min_runs[:3]), min_runs[3])
+    staged = (run_len >= thr) & (W <= MAXW)
     flags = staged.astype(np.int32) | (start.astype(np.int32) << 1)
     return key.astype(np.uint32), vis.astype(np.uint32), flags, bits
 
 
-@pytest.mark.parametrize("cfg,N,G,P,smax,seed", [(2, 30_000, 1024, 80, 72, 7),
-                                                (1, 60_000, 2048, 714, 36, 8)])
-def test_buckets_bitexact_against_selection(gpu, cfg, N, G, P, smax, seed):
+CLASS_OPTS = (_lib.OPT_MIN_STAGE_RUN_W33, _lib.OPT_MIN_STAGE_RUN_W65, _lib.OPT_MIN_STAGE_RUN_W97,
+              _lib.OPT_MIN_STAGE_RUN_W111)
+
+
+@pytest.mark.parametrize("cfg,N,G,P,smax,seed,classes", [(2, 30_000, 1024, 80, 72, 7, None),
+                                                        (1, 60_000, 2048, 714, 36, 8, None),
+                                                        (2, 30_000, 1024, 80, 72, 9, (1, 2, 6, 12))])
+def test_buckets_bitexact_against_selection(gpu, cfg, N, G, P, smax, seed, classes):
     torch, ops = gpu
     c = synth.make(cfg, num_points=N, grid_size=G, num_w_planes=P, max_support=smax, seed=seed)
     ks = ops.KernelStack(c["support"], c["oversample"], c["conv_size_half"], c["conv_func"])
@@ -73,10 +80,17 @@ def test_buckets_bitexact_against_selection(gpu, cfg, N, G, P, smax, seed):
     norm = torch.zeros(1, dtype=torch.float64, device="cuda")
     sk = torch.zeros(1, dtype=torch.int64, device="cuda")
     ctx = ops.context()
-    min_run = ctx.get_option(_lib.OPT_MIN_STAGE_RUN)
-    ops.grid(ks, d["uu"], d["vv"], d["ww"], d["vis"], d["weight"], c["cell_size_rad"],
-             c["w_scale"], G, grid, norm, sk, ctx=ctx)
-    torch.cuda.synchronize()
+    base = ctx.get_option(_lib.OPT_MIN_STAGE_RUN)
+    try:
+        for k, v in zip(CLASS_OPTS, classes or (0, 0, 0, 0)):
+            ctx.set_option(k, v)
+        ops.grid(ks, d["uu"], d["vv"], d["ww"], d["vis"], d["weight"], c["cell_size_rad"],
+                 c["w_scale"], G, grid, norm, sk, ctx=ctx)
+        torch.cuda.synchronize()
+    finally:
+        for k in CLASS_OPTS:
+            ctx.set_option(k, 0)
+    min_runs = classes or (base,) * 4
     lib = _lib.load()
     n = ctypes.c_ulonglong(0)
     bits = ctypes.c_int(0)
@@ -89,7 +103,7 @@ def test_buckets_bitexact_against_selection(gpu, cfg, N, G, P, smax, seed):
                                          ctypes.c_void_p(keys.ctypes.data),
                                          ctypes.c_void_p(vis.ctypes.data),
                                          ctypes.c_void_p(flags.ctypes.data), n.value))
-    rkey, rvis, rflags, rbits = restate(c, min_run)
+    rkey, rvis, rflags, rbits = restate(c, min_runs)
     assert bits.value == rbits
     assert n.value == len(rkey)
     assert np.array_equal(keys, rkey)

@@ -212,6 +212,31 @@ def test_kernel_variants_against_fp64_oracle(gpu, kernel, min_run):
     assert errors(g2, t)[0] <= 1e-5
 
 
+CLASS_OPTS = (_lib.OPT_MIN_STAGE_RUN_W33, _lib.OPT_MIN_STAGE_RUN_W65, _lib.OPT_MIN_STAGE_RUN_W97,
+              _lib.OPT_MIN_STAGE_RUN_W111)
+
+
+@pytest.mark.parametrize("classes", [(1, 2, 8, 12), (12, 1, 1, 2)])
+def test_per_class_stage_thresholds_against_fp64_oracle(gpu, classes):
+    """Staged-run thresholds set per support class (supports up to 72: all
+    four classes occur, plus tables too wide to stage): same bar."""
+    torch, ops = gpu
+    c = small_config(2, 30_000, 1024, 80, 72, seed=7)
+    t, tn, ts = oracle.grid_fsel_d(c)
+    ctx = ops.context()
+    try:
+        for k, v in zip(CLASS_OPTS, classes):
+            ctx.set_option(k, v)
+        assert [ctx.get_option(k) for k in CLASS_OPTS] == list(classes)
+        g, n, s = run_grid(torch, ops, c, ctx=ctx)
+    finally:
+        for k in CLASS_OPTS:
+            ctx.set_option(k, 0)
+    assert s == ts and abs(n - tn) <= 1e-12 * abs(tn)
+    emax, el2 = errors(g, t)
+    assert emax <= 1e-5 and el2 <= 1e-5, (classes, emax, el2)
+
+
 def test_device_fp64_path_against_oracle_d(gpu):
     torch, ops = gpu
     c = small_config(0, 20_000, 1024)
