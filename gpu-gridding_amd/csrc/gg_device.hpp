@@ -42,6 +42,11 @@ constexpr int kOrientMask = kFlagNegV | kFlagNegU;   // mirroring of a table rea
 // non-finite bit the first visit of a run.
 constexpr int kVisitStaged = kFlagSkip;
 constexpr int kVisitRunStart = kFlagNonFinite;
+// ... and the table-id bits (not needed past the sort) hold the table's
+// symmetry: bits 17-18 its columns, 19-20 its rows, 1 = symmetric about the
+// centre tap (offset 0: tap index |ov k|), 2 = about the half-tap before it
+// (offset ov/2, ov even: |ov/2 + ov k| = |ov/2 + ov (-1-k)|), 0 = neither.
+constexpr int kVisitFoldShift = kTableIdShift;
 // Expanded kernel tables carry this many zero rows above and below each
 // (2s+1)^2 table, so that reads of up to that many rows past a footprint
 // edge return zero taps.

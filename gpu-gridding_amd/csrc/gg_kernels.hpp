@@ -741,7 +741,8 @@ __global__ void k_run_starts(const unsigned* __restrict__ flag,
 // visits: min_run.x .. .w for tables at most 33, 65, 97 and max_w wide) and
 // kVisitRunStart where
 // the visit starts a run; y holds the visit's placement in its region (see
-// below) and vlds the visit's part of its staged LDS offsets.
+// below), the table-id bits of z the table's symmetry (kVisitFoldShift), and
+// vlds the visit's part of its staged LDS offsets.
 template <typename T>
 __global__ void k_gather_visits(const unsigned* __restrict__ vals,
         const int4* __restrict__ meta, const typename V4<T>::t* __restrict__ amp,
@@ -750,6 +751,7 @@ __global__ void k_gather_visits(const unsigned* __restrict__ vals,
         const unsigned* __restrict__ keys, int key_table_bits,
         const JobDev<T>* __restrict__ jobs, const unsigned* __restrict__ job_tile_base,
         int num_jobs, int region_h,
+        int fold_noff, int fold_half,
         const unsigned long long* __restrict__ d_n, int4* __restrict__ vmeta,
         typename V2<T>::t* __restrict__ vamp, int* __restrict__ vlds)
 {
@@ -789,6 +791,16 @@ __global__ void k_gather_visits(const unsigned* __restrict__ vals,
         const int q1 = min(top + 2 * sp, region_h - 1) >> 3;
         const unsigned cm = (2u << q1) - (1u << q0);
         m.y = (int)((cm << 16) | ((unsigned)top & 0xffffu));
+        // The table's symmetry from its id (table = (plane noff + |off_u|)
+        // noff + |off_v|; fold_noff = 0 when the ids were shortened).
+        int fold = 0;
+        if (fold_noff > 0)
+        {
+            const unsigned tab = meta_table_id(m.z) % (unsigned)(fold_noff * fold_noff);
+            const int a = (int)tab / fold_noff, b = (int)tab % fold_noff;
+            fold = (a == 0 ? 1 : a == fold_half ? 2 : 0) | ((b == 0 ? 1 : b == fold_half ? 2 : 0) << 2);
+        }
+        m.z = (m.z & ((1 << kTableIdShift) - 1)) | (fold << kVisitFoldShift);
         vmeta[p] = m;
     }
 }

@@ -947,8 +947,8 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
                 C->stream, svals, C->meta.as<int4>(), C->amp.as<T4>(), run_flag, run_excl,
                 C->run_start.as<unsigned>(), stage_thresholds(C), kStripMaxW, kStripS, skeys,
                 key_table_bits, C->jobs_d.as<JobDev<T>>(), C->job_tile_base_d.as<unsigned>(),
-                num_jobs, th, sc + SC_VISITS, C->vmeta.as<int4>(), C->vamp.as<T2>(),
-                C->vlds.as<int>());
+                num_jobs, th, table_id_shift == 0 ? noff : 0, ov % 2 == 0 ? ov / 2 : -1,
+                sc + SC_VISITS, C->vmeta.as<int4>(), C->vamp.as<T2>(), C->vlds.as<int>());
         HIP_TRY(hipGetLastError());
     }
 
