@@ -508,7 +508,8 @@ def main():
     ap.add_argument("--kernel", choices=("strips", "staged", "direct"), default="strips",
                     help="gridding kernel: 16-column strips on 128x128 regions, or 64x64 "
                          "tiles with tables staged in LDS / read directly")
-    ap.add_argument("--visits-per-item", type=int, default=4096)
+    ap.add_argument("--visits-per-item", type=int, default=0,
+                    help="tile visits per work item (0: the library's automatic choice)")
     ap.add_argument("--min-stage-run", type=int, default=3)
     ap.add_argument("--item-group", type=int, default=1,
                     help="strip kernel: consecutive work items dealt to one XCD")
@@ -722,7 +723,9 @@ def main():
                    "num_w_planes": case["num_w_planes"], "max_support": case["max_support"],
                    "oversample": case["oversample"],
                    "accumulation": "compensated" if a.compensated else "plain",
-                   "tile_kernel": a.kernel, "visits_per_item": a.visits_per_item,
+                   "tile_kernel": a.kernel,
+                   "visits_per_item": int(phases[-1]["visits_per_item"]) if phases
+                   else a.visits_per_item,
                    "min_stage_run": a.min_stage_run, "item_group": a.item_group},
         "roofline": {"bound": "mfma",
                      "bound_note": "the kernel's binding unit is the FP32 vector FMA pipe (no "

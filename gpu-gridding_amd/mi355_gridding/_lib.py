@@ -64,6 +64,7 @@ class Stats(ctypes.Structure):
         ("num_items", ctypes.c_ulonglong),
         ("num_taps", ctypes.c_ulonglong),
         ("num_merge_tiles", ctypes.c_ulonglong),
+        ("visits_per_item", ctypes.c_ulonglong),
     ]
 
     def as_dict(self):

@@ -95,7 +95,10 @@ MI355_EXPORT int mi355_ctx_set_stream(mi355_ctx* ctx, void* hip_stream);
 MI355_EXPORT int mi355_ctx_get_stream(mi355_ctx* ctx, void** hip_stream);
 
 /* Tuning and behaviour options (mi355_ctx_set_option). */
-#define MI355_OPT_VISITS_PER_ITEM 1  /* tile visits per work item (default 4096) */
+#define MI355_OPT_VISITS_PER_ITEM 1  /* tile visits per work item; 0: automatic
+                                        (default): 4096, or 2048 / 1024 for the
+                                        strip kernel when the call has fewer
+                                        than 6 x that many visibilities per CU */
 #define MI355_OPT_COMPENSATED     2  /* 1: TwoSum-compensated LDS accumulation */
 #define MI355_OPT_PHASE_TIMING    3  /* 1: record hipEvents around each phase */
 #define MI355_OPT_CACHE_KERNELS   4  /* 1: reuse prepared kernel tables while the
@@ -142,6 +145,8 @@ typedef struct
     unsigned long long num_items;   /* work items launched */
     unsigned long long num_taps;    /* sum over gridded visibilities of (2s+1)^2 */
     unsigned long long num_merge_tiles;
+    unsigned long long visits_per_item;  /* work-item size used (the option, or
+                                            the automatic choice) */
 } mi355_stats;
 
 MI355_EXPORT int mi355_ctx_get_stats(const mi355_ctx* ctx, mi355_stats* stats);

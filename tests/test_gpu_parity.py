@@ -273,6 +273,30 @@ def test_deterministic_and_item_split_invariant(gpu):
         ctx.set_option(_lib.OPT_ITEM_GROUP, g0)
 
 
+def test_automatic_visits_per_item(gpu):
+    """Option 0 (the default) picks the work-item size from the visibility
+    count: 1024 for a call of fewer than 6 x 2048 visibilities per CU; an
+    explicit size is used as given; the grid does not depend on it beyond
+    FP32 rounding (items merge in order)."""
+    torch, ops = gpu
+    c = small_config(2, 40_000, 2048, 60, 40)
+    ctx = _lib.Context(torch.cuda.current_device())
+    try:
+        assert ctx.get_option(_lib.OPT_VISITS_PER_ITEM) == 0
+        ga, na, _ = run_grid(torch, ops, c, ctx=ctx)
+        st = ctx.stats()
+        assert st["visits_per_item"] == 1024, st
+        assert st["num_items"] >= -(-st["num_visits"] // 1024)
+        gb, nb, _ = run_grid(torch, ops, c, ctx=ctx, visits_per_item=4096)
+        assert ctx.stats()["visits_per_item"] == 4096
+        assert ctx.get_option(_lib.OPT_VISITS_PER_ITEM) == 0      # (restored)
+        assert errors(ga, gb)[0] <= 1e-5 and abs(na - nb) <= 1e-12 * abs(nb)
+        with pytest.raises(_lib.GridderError):
+            ctx.set_option(_lib.OPT_VISITS_PER_ITEM, -1)
+    finally:
+        ctx.close()
+
+
 def test_device_accumulates_across_calls(gpu):
     """grid and norm accumulate, num_skipped is set per call (oskar_grid_wproj.c:151-210)."""
     torch, ops = gpu
