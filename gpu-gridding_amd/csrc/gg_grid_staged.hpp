@@ -90,23 +90,38 @@ __device__ __forceinline__ void visit_lds(const int4 mt, const typename V4<T>::t
     const bool act = k >= -s && k <= s;
     const int kcol = act ? ((mt.z & kFlagNegU) ? s - k : s + k) : S - 1;
     const bool neg_v = mt.z & kFlagNegV;
+    // Two groups of four rows are read before their multiplies (four: no
+    // faster; one: 1.5% slower for the FP64 gridder).
+    constexpr int GL = 2;
     #pragma unroll
-    for (int g = 0; g < kRowsPerWave; g += 4)
+    for (int g0 = 0; g0 < kRowsPerWave; g0 += 4 * GL)
     {
-        const int jg = j0 + g;
-        if (jg > s || jg + 3 < -s) continue;
-        T2 c[4];
-        if (!neg_v)
+        T2 c[GL][4];
+        #pragma unroll
+        for (int u = 0; u < GL; ++u)
         {
-            const T2* p = tab + (kPadRows + s + jg) * S + kcol;     // rows increase with r
-            c[0] = p[0]; c[1] = p[S]; c[2] = p[2 * S]; c[3] = p[3 * S];
+            const int g = g0 + 4 * u;
+            const int jg = j0 + g;
+            if (g >= kRowsPerWave || jg > s || jg + 3 < -s) continue;
+            if (!neg_v)
+            {
+                const T2* p = tab + (kPadRows + s + jg) * S + kcol;     // rows increase with r
+                c[u][0] = p[0]; c[u][1] = p[S]; c[u][2] = p[2 * S]; c[u][3] = p[3 * S];
+            }
+            else
+            {
+                const T2* p = tab + (kPadRows + s - jg - 3) * S + kcol; // row of r = 3 first
+                c[u][3] = p[0]; c[u][2] = p[S]; c[u][1] = p[2 * S]; c[u][0] = p[3 * S];
+            }
         }
-        else
+        #pragma unroll
+        for (int u = 0; u < GL; ++u)
         {
-            const T2* p = tab + (kPadRows + s - jg - 3) * S + kcol; // row of r = 3 first
-            c[3] = p[0]; c[2] = p[S]; c[1] = p[2 * S]; c[0] = p[3 * S];
+            const int g = g0 + 4 * u;
+            const int jg = j0 + g;
+            if (g >= kRowsPerWave || jg > s || jg + 3 < -s) continue;
+            mac4<T>(c[u], a, re + g, im + g);
         }
-        mac4<T>(c, a, re + g, im + g);
     }
 }
 
@@ -129,23 +144,40 @@ __device__ __forceinline__ void visit_direct(const int4 mt, const typename V4<T>
             ? (unsigned)((mt.z & kFlagNegU) ? s - k : s + k) * (unsigned)sizeof(T2)
             : 0x80000000u;
     const bool neg_v = mt.z & kFlagNegV;
+    // The loads of every touched group of four rows are issued before any
+    // multiply, so that a wave has all of a visit's taps in flight at once
+    // (the pass waits on memory latency: one group at a time took 27% longer
+    // for the FP64 gridder at support 72, two groups 9%).
+    constexpr int GD = kRowsPerWave / 4;
     #pragma unroll
-    for (int g = 0; g < kRowsPerWave; g += 4)
+    for (int g0 = 0; g0 < kRowsPerWave; g0 += 4 * GD)
     {
-        const int jg = j0 + g;
-        if (jg > s || jg + 3 < -s) continue;
-        T2 c[4];
+        T2 c[GD][4];
         #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int u = 0; u < GD; ++u)
         {
-            const int jr = jg + r;
-            const unsigned soff = (jr >= -s && jr <= s)
-                    ? ((unsigned)mt.w + (unsigned)((neg_v ? s - jr : s + jr) * W)) *
-                      (unsigned)sizeof(T2)
-                    : zero_row_byte;
-            c[r] = load_tap(rs, kbyte, soff, (T2*)nullptr);
+            const int g = g0 + 4 * u;
+            const int jg = j0 + g;
+            if (g >= kRowsPerWave || jg > s || jg + 3 < -s) continue;
+            #pragma unroll
+            for (int r = 0; r < 4; ++r)
+            {
+                const int jr = jg + r;
+                const unsigned soff = (jr >= -s && jr <= s)
+                        ? ((unsigned)mt.w + (unsigned)((neg_v ? s - jr : s + jr) * W)) *
+                          (unsigned)sizeof(T2)
+                        : zero_row_byte;
+                c[u][r] = load_tap(rs, kbyte, soff, (T2*)nullptr);
+            }
         }
-        mac4<T>(c, a, re + g, im + g);
+        #pragma unroll
+        for (int u = 0; u < GD; ++u)
+        {
+            const int g = g0 + 4 * u;
+            const int jg = j0 + g;
+            if (g >= kRowsPerWave || jg > s || jg + 3 < -s) continue;
+            mac4<T>(c[u], a, re + g, im + g);
+        }
     }
 }
 
