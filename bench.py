@@ -510,7 +510,8 @@ def main():
                          "tiles with tables staged in LDS / read directly")
     ap.add_argument("--visits-per-item", type=int, default=0,
                     help="tile visits per work item (0: the library's automatic choice)")
-    ap.add_argument("--min-stage-run", type=int, default=3)
+    ap.add_argument("--min-stage-run", type=int, default=0,
+                    help="shortest staged run (0: the library's automatic choice)")
     ap.add_argument("--item-group", type=int, default=1,
                     help="strip kernel: consecutive work items dealt to one XCD")
     ap.add_argument("--dropin", action="store_true",
@@ -726,7 +727,7 @@ def main():
                    "tile_kernel": a.kernel,
                    "visits_per_item": int(phases[-1]["visits_per_item"]) if phases
                    else a.visits_per_item,
-                   "min_stage_run": a.min_stage_run, "item_group": a.item_group},
+                   "min_stage_run": a.min_stage_run or "auto", "item_group": a.item_group},
         "roofline": {"bound": "mfma",
                      "bound_note": "the kernel's binding unit is the FP32 vector FMA pipe (no "
                                    "matrix instructions: the tap lookup is an indexed gather); "

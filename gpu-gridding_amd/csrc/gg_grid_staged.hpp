@@ -28,7 +28,7 @@ namespace gg {
 
 template <typename T> struct Stage;
 template <> struct Stage<float>  { static constexpr int kStride = 96; };  // tables up to 95^2
-template <> struct Stage<double> { static constexpr int kStride = 64; };  // tables up to 63^2
+template <> struct Stage<double> { static constexpr int kStride = 80; };  // tables up to 79^2
 constexpr int kPadRows = 3;
 constexpr int kStageLanes = 2;                        // waves per strip
 constexpr int kStageWaves = kWaves * kStageLanes;     // 8

@@ -108,7 +108,9 @@ MI355_EXPORT int mi355_ctx_get_stream(mi355_ctx* ctx, void** hip_stream);
                                         compensated), 2 for FP64;
                                         1: 64x64 tiles, tables read directly;
                                         2: 64x64 tiles, tables staged in LDS */
-#define MI355_OPT_MIN_STAGE_RUN   6  /* shortest same-table run staged in LDS (default 3) */
+#define MI355_OPT_MIN_STAGE_RUN   6  /* shortest same-table run staged in LDS;
+                                        0: automatic (default): 3, and 8 for the
+                                        FP64 (64x64) kernel */
 #define MI355_OPT_ITEM_GROUP      7  /* strip kernel: work items are dealt to the
                                         8 XCDs in groups of this many consecutive
                                         items (default 1: round-robin) */

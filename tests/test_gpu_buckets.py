@@ -80,7 +80,7 @@ def test_buckets_bitexact_against_selection(gpu, cfg, N, G, P, smax, seed, class
     norm = torch.zeros(1, dtype=torch.float64, device="cuda")
     sk = torch.zeros(1, dtype=torch.int64, device="cuda")
     ctx = ops.context()
-    base = ctx.get_option(_lib.OPT_MIN_STAGE_RUN)
+    base = ctx.get_option(_lib.OPT_MIN_STAGE_RUN) or 3    # (0: automatic, 3 for this kernel)
     try:
         for k, v in zip(CLASS_OPTS, classes or (0, 0, 0, 0)):
             ctx.set_option(k, v)

@@ -246,6 +246,26 @@ def test_device_fp64_path_against_oracle_d(gpu):
     assert errors(g, t)[0] <= 1e-12
 
 
+@pytest.mark.parametrize("min_run", [1, 0])
+def test_device_fp64_staging_widths_against_oracle_d(gpu, min_run):
+    """FP64 through the 64x64 kernel with supports up to 40 (tables up to 81
+    wide: those up to 79 fit its staging stride, 81 is read directly), every
+    run staged (min run 1) or the automatic threshold: <= 1e-12 against the
+    serial FP64 oracle."""
+    torch, ops = gpu
+    c = small_config(2, 30_000, 1024, 80, 40, seed=11)
+    t, tn, ts = oracle.grid_d(c)
+    ctx = ops.context()
+    msr0 = ctx.get_option(_lib.OPT_MIN_STAGE_RUN)
+    ctx.set_option(_lib.OPT_MIN_STAGE_RUN, min_run)
+    try:
+        g, n, s = run_grid(torch, ops, c, dtype=torch.float64, ctx=ctx)
+    finally:
+        ctx.set_option(_lib.OPT_MIN_STAGE_RUN, msr0)
+    assert s == ts and abs(n - tn) <= 1e-12 * abs(tn)
+    assert errors(g, t)[0] <= 1e-12
+
+
 def test_deterministic_and_item_split_invariant(gpu):
     torch, ops = gpu
     c = small_config(2, 40_000, 2048, 60, 40)
