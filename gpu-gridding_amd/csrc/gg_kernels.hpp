@@ -587,9 +587,15 @@ void k_sort_hist(const unsigned* __restrict__ keys,
     }
 }
 
-// Stable scatter: within a chunk, elements are ranked in index order per
-// digit (wave-level match by ballots, then a prefix over the four waves),
-// and the chunk's base for each digit comes from the digit-major scan.
+// Stable scatter.  Wave w of a chunk takes its 1024 consecutive elements
+// (w * 1024 ..), 64 at a time: lanes whose keys share a digit are matched
+// with BITS ballots, and each element's rank among the wave's earlier
+// elements of its digit comes from a running per-wave count in LDS (the
+// lowest lane of each digit group advances it).  One prefix over the waves
+// per digit then gives each wave's base, so that ranks follow index order
+// and the chunk's base for each digit comes from the digit-major scan.  The
+// keys, values and ranks stay in registers meanwhile: two barriers per
+// chunk.
 template <int BITS>
 __global__ __launch_bounds__(kSortThreads)
 void k_sort_scatter(const unsigned* __restrict__ keys_in,
@@ -600,32 +606,39 @@ void k_sort_scatter(const unsigned* __restrict__ keys_in,
 {
     constexpr int R = 1 << BITS;
     constexpr unsigned M = R - 1;
+    constexpr int NW = kSortThreads / 64;
+    constexpr int SUB = kSortChunk / NW / 64;          // 64-element rows per wave
     __shared__ unsigned goff[R];
-    __shared__ unsigned run[R];
-    __shared__ unsigned wcnt[kSortThreads / 64][R];
-    __shared__ unsigned wbase[kSortThreads / 64][R];
+    __shared__ unsigned wcnt[NW][R];
     const unsigned long long n = *d_n;
     const unsigned long long nchunk = (n + kSortChunk - 1) / kSortChunk;
     const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
     const unsigned long long lt = lanemask_lt();
     for (unsigned long long c = blockIdx.x; c < nchunk; c += gridDim.x)
     {
         for (int d = threadIdx.x; d < R; d += kSortThreads)
         {
             goff[d] = hist_off[d * nchunk + c];
-            run[d] = 0;
             #pragma unroll
-            for (int w = 0; w < kSortThreads / 64; ++w) wcnt[w][d] = 0;
+            for (int w = 0; w < NW; ++w) wcnt[w][d] = 0;
         }
         __syncthreads();
-        const unsigned long long base = c * kSortChunk;
-        for (int r = 0; r < kSortPerThread; ++r)
+        const unsigned long long base = c * kSortChunk + (unsigned long long)wave * (SUB * 64);
+        unsigned key[SUB], val[SUB], rank[SUB];
+        #pragma unroll
+        for (int r = 0; r < SUB; ++r)
         {
-            const unsigned long long idx = base + r * kSortThreads + threadIdx.x;
+            const unsigned long long idx = base + r * 64 + lane;
             const bool valid = idx < n;
-            const unsigned key = valid ? keys_in[idx] : 0u;
-            const unsigned val = valid ? vals_in[idx] : 0u;
-            const unsigned d = (key >> shift) & M;
+            key[r] = valid ? keys_in[idx] : 0u;
+            val[r] = valid ? vals_in[idx] : 0u;
+        }
+        #pragma unroll
+        for (int r = 0; r < SUB; ++r)
+        {
+            const bool valid = base + r * 64 + lane < n;
+            const unsigned d = (key[r] >> shift) & M;
             unsigned long long peers = __ballot(valid);
             #pragma unroll
             for (int b = 0; b < BITS; ++b)
@@ -634,27 +647,32 @@ void k_sort_scatter(const unsigned* __restrict__ keys_in,
                 const unsigned long long bal = __ballot(bit);
                 peers &= bit ? bal : ~bal;
             }
-            const unsigned rank = (unsigned)__popcll(peers & lt);
-            if (valid && rank == 0) wcnt[wave][d] = (unsigned)__popcll(peers);
-            __syncthreads();
-            for (int e = threadIdx.x; e < R; e += kSortThreads)
+            const unsigned before = (unsigned)__popcll(peers & lt);
+            const unsigned prior = valid ? wcnt[wave][d] : 0u;
+            rank[r] = prior + before;
+            if (valid && before == 0) wcnt[wave][d] = prior + (unsigned)__popcll(peers);
+        }
+        __syncthreads();
+        for (int d = threadIdx.x; d < R; d += kSortThreads)
+        {
+            unsigned acc = goff[d];
+            #pragma unroll
+            for (int w = 0; w < NW; ++w)
             {
-                unsigned acc = run[e];
-                #pragma unroll
-                for (int w = 0; w < kSortThreads / 64; ++w)
-                {
-                    wbase[w][e] = acc;
-                    acc += wcnt[w][e];
-                    wcnt[w][e] = 0;
-                }
-                run[e] = acc;
+                const unsigned t = wcnt[w][d];
+                wcnt[w][d] = acc;
+                acc += t;
             }
-            __syncthreads();
-            if (valid)
+        }
+        __syncthreads();
+        #pragma unroll
+        for (int r = 0; r < SUB; ++r)
+        {
+            if (base + r * 64 + lane < n)
             {
-                const unsigned dst = goff[d] + wbase[wave][d] + rank;
-                keys_out[dst] = key;
-                vals_out[dst] = val;
+                const unsigned dst = wcnt[wave][(key[r] >> shift) & M] + rank[r];
+                keys_out[dst] = key[r];
+                vals_out[dst] = val[r];
             }
         }
         __syncthreads();
