@@ -115,6 +115,16 @@ constexpr int kSelThreads = 256;
 constexpr int kSelPerThread = 4;
 constexpr int kSelChunk = kSelThreads * kSelPerThread;   // visibilities per chunk
 
+// A visibility's record and weighted amplitude side by side (32 bytes), for
+// the strip kernel's gather, which reads them in sorted-visit order, that is
+// at random: one 32-byte read instead of two 16-byte ones.
+struct __align__(32) VisRec
+{
+    int4 meta;
+    float2 amp;                      // weight * vis (re, im)
+    float2 pad;
+};
+
 template <typename TI, typename TA>
 struct SelectArgs
 {
@@ -127,6 +137,7 @@ struct SelectArgs
     int tw, th;                      // bucketing tile width / height (cells)
     // outputs
     int4* meta; typename V4<TA>::t* amp; unsigned* num_visits;
+    VisRec* vrec;                    // FP32 strip path: meta and (re, im) in one record, or null
     double* part_norm; unsigned long long* part_skipped;
     unsigned long long* part_taps; unsigned long long* part_nonfinite; int4* part_bbox;
 };
@@ -220,6 +231,7 @@ void k_select(const SelectArgs<TI, TA> A, unsigned num_chunks)
             int flags = (s.off_v < 0 ? kFlagNegV : 0) |
                     (s.off_u < 0 ? kFlagNegU : 0) | (s.conj ? kFlagConj : 0);
             const TI w_in = A.weight[i];
+            TA amp_re = (TA)0, amp_im = (TA)0;
             if (A.vis)
             {
                 // Amplitude: weight * vis (oskar_grid_wproj.c:166-169) in the
@@ -234,6 +246,8 @@ void k_select(const SelectArgs<TI, TA> A, unsigned num_chunks)
                 am.z = v_im;
                 am.w = s.conj ? -v_re : v_re;
                 A.amp[i] = am;
+                amp_re = v_re;
+                amp_im = v_im;
                 if (!isfinite(v_re) || !isfinite(v_im))
                 {
                     flags |= kFlagNonFinite;
@@ -245,6 +259,17 @@ void k_select(const SelectArgs<TI, TA> A, unsigned num_chunks)
             m.w = (int)(A.plane_off[s.plane] + tab * (unsigned)(W + 2 * kTablePadRows) * (unsigned)W +
                     (unsigned)(kTablePadRows * W));
             A.meta[i] = m;
+            if constexpr (std::is_same<TA, float>::value)
+            {
+                if (A.vrec)
+                {
+                    VisRec rr;
+                    rr.meta = m;
+                    rr.amp = make_float2(amp_re, amp_im);
+                    rr.pad = make_float2(0.f, 0.f);
+                    A.vrec[i] = rr;
+                }
+            }
             // Tile visits (none for non-finite amplitudes: k_grid_nonfinite
             // grids those after the tiles are done).
             // (footprints of gridded visibilities lie inside the grid: the
@@ -763,7 +788,7 @@ __global__ void k_run_starts(const unsigned* __restrict__ flag,
 // vlds the visit's part of its staged LDS offsets.
 template <typename T>
 __global__ void k_gather_visits(const unsigned* __restrict__ vals,
-        const int4* __restrict__ meta, const typename V4<T>::t* __restrict__ amp,
+        const VisRec* __restrict__ vrec,
         const unsigned* __restrict__ run_flag, const unsigned* __restrict__ run_excl,
         const unsigned* __restrict__ run_start, uint4 min_run, int max_w, int lds_stride,
         const unsigned* __restrict__ keys, int key_table_bits,
@@ -778,15 +803,15 @@ __global__ void k_gather_visits(const unsigned* __restrict__ vals,
             p < n; p += (unsigned long long)gridDim.x * blockDim.x)
     {
         const unsigned i = vals[p];
-        int4 m = meta[i];
+        const VisRec rec = vrec[i];
+        int4 m = rec.meta;
         const unsigned r = run_excl[p] + run_flag[p] - 1u;
         const int W = 2 * meta_support(m.z) + 1;
         const unsigned mr = W <= 33 ? min_run.x : W <= 65 ? min_run.y : W <= 97 ? min_run.z
                                                                            : min_run.w;
         if (run_start[r + 1] - run_start[r] >= mr && W <= max_w) m.z |= kVisitStaged;
         if (run_flag[p]) m.z |= kVisitRunStart;
-        const typename V4<T>::t am = amp[i];
-        vamp[p] = {am.x, am.z};            // the weighted amplitude (re, im)
+        vamp[p] = {rec.amp.x, rec.amp.y};  // the weighted amplitude (re, im)
         // The visit's part of its staged-tap LDS byte offsets (the strip
         // kernel adds the lane's part): footprint column 0 is table column
         // s - x + (strip column), or x + s - (strip column) mirrored in u;

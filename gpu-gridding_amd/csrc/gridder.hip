@@ -240,6 +240,7 @@ struct mi355_ctx
     DevBuf tile_start, tile_end, n_items, n_slots, item_off, slot_off, item_tile;
     DevBuf run_start, run_z, run_tb;
     DevBuf vmeta, vamp, vlds;       // sorted visit records (strip kernel)
+    DevBuf vrec;                    // per-visibility record + amplitude (strip kernel's gather)
     DevBuf scratch;
     DevBuf jobs_d, job_chunk_off_d, job_vis_off_d, job_tile_base_d, job_out_d, job_bbox_d, job_norm_d;
     // drop-in staging
@@ -448,7 +449,7 @@ std::vector<DevBuf*> device_buffers(mi355_ctx* C)
         &C->jobs_d, &C->job_chunk_off_d, &C->job_vis_off_d, &C->job_tile_base_d,
         &C->job_out_d, &C->job_bbox_d, &C->job_norm_d, &C->in_conv, &C->in_uu, &C->in_vv, &C->in_ww,
         &C->in_vis, &C->in_weight, &C->in_grid, &C->in_norm, &C->in_skipped, &C->in_weight_out,
-        &C->fft_work, &C->in_image, &C->vmeta, &C->vamp, &C->vlds};
+        &C->fft_work, &C->in_image, &C->vmeta, &C->vamp, &C->vlds, &C->vrec};
 }
 
 int get_default_ctx(mi355_ctx** out)
@@ -817,6 +818,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     // Selection.
     TRY(ensure(C, C->meta, n_total * sizeof(int4)));
     TRY(ensure(C, C->amp, n_total * sizeof(T4)));
+    if (strips) TRY(ensure(C, C->vrec, n_total * sizeof(VisRec)));
     TRY(ensure(C, C->num_visits, n_total * sizeof(unsigned)));
     TRY(ensure(C, C->visit_off, (n_total + 1) * sizeof(unsigned)));
     TRY(ensure(C, C->part_norm, chunks * sizeof(double)));
@@ -840,6 +842,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
     sa.table_id_shift = table_id_shift;
     sa.tw = tw; sa.th = th;
     sa.meta = C->meta.as<int4>(); sa.amp = C->amp.as<T4>();
+    sa.vrec = strips ? C->vrec.as<VisRec>() : nullptr;
     sa.num_visits = C->num_visits.as<unsigned>();
     sa.part_norm = C->part_norm.as<double>();
     sa.part_skipped = C->part_skipped.as<unsigned long long>();
@@ -971,7 +974,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
         TRY(ensure(C, C->vamp, T_vis * sizeof(T2)));
         TRY(ensure(C, C->vlds, T_vis * sizeof(int)));
         hipLaunchKernelGGL(k_gather_visits<T>, dim3(grid_blocks(T_vis, 256, 65535)), dim3(256), 0,
-                C->stream, svals, C->meta.as<int4>(), C->amp.as<T4>(), run_flag, run_excl,
+                C->stream, svals, C->vrec.as<VisRec>(), run_flag, run_excl,
                 C->run_start.as<unsigned>(), stage_thresholds(C), kStripMaxW, kStripS, skeys,
                 key_table_bits, C->jobs_d.as<JobDev<T>>(), C->job_tile_base_d.as<unsigned>(),
                 num_jobs, th, table_id_shift == 0 ? noff : 0, ov % 2 == 0 ? ov / 2 : -1,
