@@ -44,8 +44,9 @@ constexpr int kStripW = 16;                        // columns per wave
 // CU: the staged table takes the LDS), each owning 16 columns x ROWS rows,
 // and COMP for compensated accumulation.
 //  - Plain FP32: 16 waves (four per SIMD, up to 128 VGPRs, 64 of them
-//    accumulators) on 256 x 128 regions, 2 chunks of global taps loaded
-//    together.  Four waves per SIMD hide more of the staged pass's LDS
+//    accumulators) on 256 x 128 regions, 8 chunks of global taps loaded
+//    together (2, 3, 4: +4.9%, +3.2%, +0.7%; 16 spills).  Four waves per
+//    SIMD hide more of the staged pass's LDS
 //    latency than three at 12 waves (8 chunks, up to 168 VGPRs): the staged
 //    pass alone took 53.7 ms against 60.5 ms, for a slightly slower direct
 //    pass.
@@ -65,7 +66,7 @@ struct StripCfg
     static constexpr int kSlots = ROWS / 4;          // slots of 16 columns x 4 rows
     static constexpr int kChunks = ROWS / 8;         // chunks of 8 rows
 #ifndef GG_DIRECT_GROUP
-#define GG_DIRECT_GROUP (WAVES >= 16 ? 2 : 8)
+#define GG_DIRECT_GROUP 8
 #endif
 #ifndef GG_DIRECT_GROUP_COMP
 #define GG_DIRECT_GROUP_COMP 8
