@@ -282,19 +282,18 @@ mi355_ctx* g_default[64] = {};
 
 // The strip kernel's staged-run thresholds per support class (tables at
 // most 33, 65, 97 and 111 wide).
-// Automatic visits per work item: 4096, or 2048 / 1024 for the strip kernel
-// when the call has fewer than 6 such item sizes of visibilities per CU
-// (about 8 items per CU at 1.5 tile visits per visibility), so that the
-// items of its heaviest regions spread over more CUs: at 1M and 2.15M
-// visibilities the kernel took 28% and 15% less time with 1024 than with
-// 4096.  The choice depends on the visibility count only, so the
-// host-sync-free form (which knows the tile-visit count only as a bound)
-// splits the work exactly as the synchronous one does.
-unsigned auto_visits_per_item(unsigned long long num_points, int num_cu, bool strips)
+// Automatic visits per work item: 4096, or 2048 / 1024 when the call has
+// fewer than 6 such item sizes of visibilities per CU (about 8 items per
+// CU at 1.5 tile visits per visibility), so that the items of its heaviest
+// regions spread over more CUs: at 1M and 2.15M visibilities the strip
+// kernel took 28% and 15% less time with 1024 than with 4096, the FP64
+// 64x64 kernel 17% and 4%.  The choice depends on the visibility count
+// only, so the host-sync-free form (which knows the tile-visit count only
+// as a bound) splits the work exactly as the synchronous one does.
+unsigned auto_visits_per_item(unsigned long long num_points, int num_cu)
 {
     unsigned v = 4096;
-    if (strips)
-        while (v > 1024 && num_points < 6ull * v * (unsigned long long)std::max(num_cu, 1)) v >>= 1;
+    while (v > 1024 && num_points < 6ull * v * (unsigned long long)std::max(num_cu, 1)) v >>= 1;
     return v;
 }
 
@@ -918,7 +917,7 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
 
     // Visits per work item: the option, or automatic.
     const unsigned vpi = C->visits_per_item ? C->visits_per_item
-                                            : auto_visits_per_item(n_total, C->num_cu, strips);
+                                            : auto_visits_per_item(n_total, C->num_cu);
     C->stats.visits_per_item = vpi;
 
     // Emit (tile | table) keys and sort them.
