@@ -109,8 +109,10 @@ MI355_EXPORT int mi355_ctx_get_stream(mi355_ctx* ctx, void** hip_stream);
                                         1: 64x64 tiles, tables read directly;
                                         2: 64x64 tiles, tables staged in LDS */
 #define MI355_OPT_MIN_STAGE_RUN   6  /* shortest same-table run staged in LDS;
-                                        0: automatic (default): 3, and 8 for the
-                                        FP64 (64x64) kernel */
+                                        0: automatic (default): for the strip
+                                        kernel 6 for tables up to 65 wide and 3
+                                        for wider ones; 3 for the 64x64 kernels,
+                                        8 for the FP64 one */
 #define MI355_OPT_ITEM_GROUP      7  /* strip kernel: work items are dealt to the
                                         8 XCDs in groups of this many consecutive
                                         items (default 1: round-robin) */

@@ -80,7 +80,7 @@ def test_buckets_bitexact_against_selection(gpu, cfg, N, G, P, smax, seed, class
     norm = torch.zeros(1, dtype=torch.float64, device="cuda")
     sk = torch.zeros(1, dtype=torch.int64, device="cuda")
     ctx = ops.context()
-    base = ctx.get_option(_lib.OPT_MIN_STAGE_RUN) or 3    # (0: automatic, 3 for this kernel)
+    base = ctx.get_option(_lib.OPT_MIN_STAGE_RUN)
     try:
         for k, v in zip(CLASS_OPTS, classes or (0, 0, 0, 0)):
             ctx.set_option(k, v)
@@ -90,7 +90,8 @@ def test_buckets_bitexact_against_selection(gpu, cfg, N, G, P, smax, seed, class
     finally:
         for k in CLASS_OPTS:
             ctx.set_option(k, 0)
-    min_runs = classes or (base,) * 4
+    # (option 0: the automatic thresholds, 6 for tables up to 65 wide, 3 above)
+    min_runs = classes or ((base,) * 4 if base else (6, 6, 3, 3))
     lib = _lib.load()
     n = ctypes.c_ulonglong(0)
     bits = ctypes.c_int(0)
