@@ -282,17 +282,20 @@ mi355_ctx* g_default[64] = {};
 
 // The strip kernel's staged-run thresholds per support class (tables at
 // most 33, 65, 97 and 111 wide).
-// Automatic visits per work item: 4096, or 2048 / 1024 when the call has
-// fewer than 6 such item sizes of visibilities per CU (about 8 items per
-// CU at 1.5 tile visits per visibility), so that the items of its heaviest
-// regions spread over more CUs: at 1M and 2.15M visibilities the strip
-// kernel took 28% and 15% less time with 1024 than with 4096, the FP64
-// 64x64 kernel 17% and 4%.  The choice depends on the visibility count
-// only, so the host-sync-free form (which knows the tile-visit count only
-// as a bound) splits the work exactly as the synchronous one does.
+// Automatic visits per work item: the largest of 8192, 4096, 2048, 1024
+// for which the call has at least 6 item sizes of visibilities per CU
+// (about 8 items per CU at 1.5 tile visits per visibility), else 1024:
+// fewer, larger items mean fewer partial regions to merge (31.4M
+// visibilities: 8192 took 78.1 ms per pass against 78.8 with 4096), and
+// small calls need small items to spread their heaviest regions over the
+// CUs (at 1M and 2.15M visibilities the strip kernel took 28% and 15% less
+// time with 1024 than with 4096, the FP64 64x64 kernel 17% and 4%).  The
+// choice depends on the visibility count only, so the host-sync-free form
+// (which knows the tile-visit count only as a bound) splits the work
+// exactly as the synchronous one does.
 unsigned auto_visits_per_item(unsigned long long num_points, int num_cu)
 {
-    unsigned v = 4096;
+    unsigned v = 8192;
     while (v > 1024 && num_points < 6ull * v * (unsigned long long)std::max(num_cu, 1)) v >>= 1;
     return v;
 }

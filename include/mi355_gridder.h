@@ -96,9 +96,9 @@ MI355_EXPORT int mi355_ctx_get_stream(mi355_ctx* ctx, void** hip_stream);
 
 /* Tuning and behaviour options (mi355_ctx_set_option). */
 #define MI355_OPT_VISITS_PER_ITEM 1  /* tile visits per work item; 0: automatic
-                                        (default): 4096, or 2048 / 1024 when the
-                                        call has fewer than 6 x that many
-                                        visibilities per CU */
+                                        (default): the largest of 8192, 4096,
+                                        2048 for which the call has at least 6 x
+                                        that many visibilities per CU, else 1024 */
 #define MI355_OPT_COMPENSATED     2  /* 1: TwoSum-compensated LDS accumulation */
 #define MI355_OPT_PHASE_TIMING    3  /* 1: record hipEvents around each phase */
 #define MI355_OPT_CACHE_KERNELS   4  /* 1: reuse prepared kernel tables while the
