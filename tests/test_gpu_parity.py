@@ -351,6 +351,28 @@ def test_grid_sizes_off_the_region_grid(gpu, G, smax):
         assert emax <= 1e-5 and el2 <= 1e-5, (mode, emax, el2)
 
 
+def test_many_planes_shortened_table_ids_against_fp64_oracle(gpu):
+    """4000 w-planes at oversample 4: 36000 kernel tables, more than the 15
+    bits of table id a visit record holds, so the ids are shortened for the
+    sort (runs then compare table bases) and the direct reads do not fold
+    symmetric tables.  Strip kernel and 64 x 64 staged kernel against the
+    FP64 truth; num_skipped bit-exact, norm to 1e-12."""
+    torch, ops = gpu
+    c = small_config(2, 20_000, 512, 4000, 8, seed=5)
+    assert len(c["support"]) == 4000
+    t, tn, ts = oracle.grid_fsel_d(c)
+    ctx = ops.context()
+    try:
+        for kernel in (0, 2):
+            ctx.set_option(_lib.OPT_GRID_KERNEL, kernel)
+            g, n, s = run_grid(torch, ops, c, ctx=ctx)
+            assert s == ts and abs(n - tn) <= 1e-12 * abs(tn), (kernel, s, ts, n, tn)
+            emax, el2 = errors(g, t)
+            assert emax <= 1e-5 and el2 <= 1e-5, (kernel, emax, el2)
+    finally:
+        ctx.set_option(_lib.OPT_GRID_KERNEL, 0)
+
+
 @pytest.mark.parametrize("smax,N", [(100, 4000), (160, 3000)])
 def test_large_supports_against_fp64_oracle(gpu, smax, N):
     """Supports beyond the shapes above (16 planes, 4 .. smax): footprints up
