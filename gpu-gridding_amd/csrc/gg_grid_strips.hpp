@@ -406,8 +406,8 @@ void k_grid_strips(const GridArgs<float> A)
         // once (about a third less traffic than free-running waves; without
         // it the kernel took 8% longer at support 72).  A batch that holds
         // one of support 49 or more is taken in quarters of 16 visits with a
-        // barrier before each (1.9% faster than one barrier per batch at
-        // support 72, halves 1.7%).  Smaller tables stay in the caches, and
+        // barrier before each (plain kernel: 1.9% faster than one barrier per
+        // batch at support 72, halves 1.7%).  Smaller tables stay in the caches, and
         // there the barrier only costs (8% at support 36).  (Every wave holds
         // the batch's records: the tests are uniform.)
         for (unsigned b = v0; b < v1; b += 64)
@@ -417,7 +417,7 @@ void k_grid_strips(const GridArgs<float> A)
             const unsigned long long todo_batch = batch_mask(b, v1, mv, av, false);
             const bool direct = b + (unsigned)lane < v1 && !(mv.z & kVisitStaged);
             const bool sync = __ballot(direct && meta_support(mv.z) >= 33) != 0ull;
-            const int parts = __ballot(direct && meta_support(mv.z) >= 49) ? 4 : 1;
+            const int parts = !kComp && __ballot(direct && meta_support(mv.z) >= 49) ? 4 : 1;
             for (int part = 0; part < parts; ++part)
             {
             if (part > 0 || sync) __syncthreads();
