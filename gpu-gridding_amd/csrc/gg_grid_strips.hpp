@@ -420,88 +420,88 @@ void k_grid_strips(const GridArgs<float> A)
             const int parts = !kComp && __ballot(direct && meta_support(mv.z) >= 49) ? 4 : 1;
             for (int part = 0; part < parts; ++part)
             {
-            if (part > 0 || sync) __syncthreads();
-            unsigned long long todo = parts == 1 ? todo_batch : todo_batch & (0xffffull << (16 * part));
-            while (todo)
-            {
-                const int l = __builtin_ctzll(todo);
-                todo &= todo - 1ull;
-                GG_VISIT_HEAD
-                const int tb = bcast(mv.w, l);
-                const int W = 2 * s + 1;
-                const bool neg_v = z & kFlagNegV, neg_u = z & kFlagNegU;
-                const int fu = (z >> kVisitFoldShift) & 3, fv = (z >> (kVisitFoldShift + 2)) & 3;
-                // Lane's footprint column and its table column.  A table
-                // symmetric in its columns is read in its right half only:
-                // column c < s reads column 2s - c (fold 1: symmetric about
-                // the centre) or 2s - 1 - c (fold 2: about the half-column
-                // before it), which holds the same tap, so that a visit
-                // fetches fewer bytes and the strips either side of the
-                // centre share lines.
-                const int k = kc + lc;
-                int kt = neg_u ? W - 1 - k : k;
-                if (fu && kt < s) kt = (2 * s + 1 - fu) - kt;
-                const unsigned vcol = (k >= 0 && k < W) ? (unsigned)kt * 8u : 0x80000000u;
-                // Rows: the lane's table row in slot i is t = t0 + i dt (rows
-                // run backwards when mirrored).  A table symmetric about its
-                // centre row (fold 1) reads row max(t, 2s - t), the same tap;
-                // rows above and below the footprint (t in -3..-1 and W..W+2:
-                // slots end at most 3 rows past it) stay zero pad rows.  (A
-                // table symmetric about the half-row before its centre pairs
-                // rows t and 2s-1-t but leaves row 2s unpaired: folding it
-                // would cost more per slot than it saves.)  The table's byte
-                // offsets ascend with t, so the row read is the larger of two
-                // offsets, one add each per slot: oa + i da (row t) and
-                // ob + i db (row 2s - t, or row t again when the rows do not
-                // fold).  Offsets with vcol out of range are at least 2^31,
-                // negative as ints, and the signed max of two such is one of
-                // them: still out of range.
-                const int t0 = neg_v ? W - 1 + top - lr : lr - top;
-                const unsigned vtb = vcol + (unsigned)tb * 8u;
-                const int da = (neg_v ? -32 : 32) * W;
-                const unsigned oa = vtb + (unsigned)(t0 * 8 * W);
-                const unsigned ob = fv == 1 ? vtb + (unsigned)((2 * s - t0) * 8 * W) : oa;
-                const int db = fv == 1 ? -da : da;
-                // slots (4 rows) that hold footprint rows: only those are read
-                const int i0 = max(top, 0) >> 2;
-                const int i1 = min(top + 2 * s, kStripRows - 1) >> 2;
-                const unsigned sm = (unsigned)((2ull << i1) - (1ull << i0));
-                // Touched chunks' taps are requested kDirectGroup chunks at a
-                // time before their multiplies: up to 2 * kDirectGroup loads in
-                // flight.  slot_off(i): the byte offset of the lane's tap in
-                // slot i.
-                auto chunks = [&](auto slot_off) {
-                    #pragma unroll
-                    for (int h = 0; h < kStripChunks; h += kDirectGroup)
-                    {
-                        if (!((cm >> h) & ((1 << kDirectGroup) - 1))) continue;
-                        float2 tp[kDirectGroup][2];
+                if (part > 0 || sync) __syncthreads();
+                unsigned long long todo = parts == 1 ? todo_batch : todo_batch & (0xffffull << (16 * part));
+                while (todo)
+                {
+                    const int l = __builtin_ctzll(todo);
+                    todo &= todo - 1ull;
+                    GG_VISIT_HEAD
+                    const int tb = bcast(mv.w, l);
+                    const int W = 2 * s + 1;
+                    const bool neg_v = z & kFlagNegV, neg_u = z & kFlagNegU;
+                    const int fu = (z >> kVisitFoldShift) & 3, fv = (z >> (kVisitFoldShift + 2)) & 3;
+                    // Lane's footprint column and its table column.  A table
+                    // symmetric in its columns is read in its right half only:
+                    // column c < s reads column 2s - c (fold 1: symmetric about
+                    // the centre) or 2s - 1 - c (fold 2: about the half-column
+                    // before it), which holds the same tap, so that a visit
+                    // fetches fewer bytes and the strips either side of the
+                    // centre share lines.
+                    const int k = kc + lc;
+                    int kt = neg_u ? W - 1 - k : k;
+                    if (fu && kt < s) kt = (2 * s + 1 - fu) - kt;
+                    const unsigned vcol = (k >= 0 && k < W) ? (unsigned)kt * 8u : 0x80000000u;
+                    // Rows: the lane's table row in slot i is t = t0 + i dt (rows
+                    // run backwards when mirrored).  A table symmetric about its
+                    // centre row (fold 1) reads row max(t, 2s - t), the same tap;
+                    // rows above and below the footprint (t in -3..-1 and W..W+2:
+                    // slots end at most 3 rows past it) stay zero pad rows.  (A
+                    // table symmetric about the half-row before its centre pairs
+                    // rows t and 2s-1-t but leaves row 2s unpaired: folding it
+                    // would cost more per slot than it saves.)  The table's byte
+                    // offsets ascend with t, so the row read is the larger of two
+                    // offsets, one add each per slot: oa + i da (row t) and
+                    // ob + i db (row 2s - t, or row t again when the rows do not
+                    // fold).  Offsets with vcol out of range are at least 2^31,
+                    // negative as ints, and the signed max of two such is one of
+                    // them: still out of range.
+                    const int t0 = neg_v ? W - 1 + top - lr : lr - top;
+                    const unsigned vtb = vcol + (unsigned)tb * 8u;
+                    const int da = (neg_v ? -32 : 32) * W;
+                    const unsigned oa = vtb + (unsigned)(t0 * 8 * W);
+                    const unsigned ob = fv == 1 ? vtb + (unsigned)((2 * s - t0) * 8 * W) : oa;
+                    const int db = fv == 1 ? -da : da;
+                    // slots (4 rows) that hold footprint rows: only those are read
+                    const int i0 = max(top, 0) >> 2;
+                    const int i1 = min(top + 2 * s, kStripRows - 1) >> 2;
+                    const unsigned sm = (unsigned)((2ull << i1) - (1ull << i0));
+                    // Touched chunks' taps are requested kDirectGroup chunks at a
+                    // time before their multiplies: up to 2 * kDirectGroup loads in
+                    // flight.  slot_off(i): the byte offset of the lane's tap in
+                    // slot i.
+                    auto chunks = [&](auto slot_off) {
                         #pragma unroll
-                        for (int u = 0; u < kDirectGroup; ++u)
+                        for (int h = 0; h < kStripChunks; h += kDirectGroup)
                         {
-                            const int q = h + u;
-                            if (q >= kStripChunks) continue;
-                            if (sm & (1u << (2 * q)))
-                                tp[u][0] = load_tap(rs_tab, slot_off(2 * q), 0u, (float2*)nullptr);
-                            if (sm & (2u << (2 * q)))
-                                tp[u][1] = load_tap(rs_tab, slot_off(2 * q + 1), 0u, (float2*)nullptr);
+                            if (!((cm >> h) & ((1 << kDirectGroup) - 1))) continue;
+                            float2 tp[kDirectGroup][2];
+                            #pragma unroll
+                            for (int u = 0; u < kDirectGroup; ++u)
+                            {
+                                const int q = h + u;
+                                if (q >= kStripChunks) continue;
+                                if (sm & (1u << (2 * q)))
+                                    tp[u][0] = load_tap(rs_tab, slot_off(2 * q), 0u, (float2*)nullptr);
+                                if (sm & (2u << (2 * q)))
+                                    tp[u][1] = load_tap(rs_tab, slot_off(2 * q + 1), 0u, (float2*)nullptr);
+                            }
+                            #pragma unroll
+                            for (int u = 0; u < kDirectGroup; ++u)
+                            {
+                                const int q = h + u;
+                                if (q >= kStripChunks) continue;
+                                if (sm & (1u << (2 * q))) mac_strip(re[2 * q], im[2 * q], tp[u][0], a);
+                                if (sm & (2u << (2 * q)))
+                                    mac_strip(re[2 * q + 1], im[2 * q + 1], tp[u][1], a);
+                            }
                         }
-                        #pragma unroll
-                        for (int u = 0; u < kDirectGroup; ++u)
-                        {
-                            const int q = h + u;
-                            if (q >= kStripChunks) continue;
-                            if (sm & (1u << (2 * q))) mac_strip(re[2 * q], im[2 * q], tp[u][0], a);
-                            if (sm & (2u << (2 * q)))
-                                mac_strip(re[2 * q + 1], im[2 * q + 1], tp[u][1], a);
-                        }
-                    }
-                };
-                chunks([&](int i) {
-                    return (unsigned)max((int)(oa + (unsigned)(i * da)), (int)(ob + (unsigned)(i * db)));
-                });
-                visit_done();
-            }
+                    };
+                    chunks([&](int i) {
+                        return (unsigned)max((int)(oa + (unsigned)(i * da)), (int)(ob + (unsigned)(i * db)));
+                    });
+                    visit_done();
+                }
             }
         }
         }
