@@ -100,7 +100,9 @@ static_assert(kStripGuard + (kStripMaxW + 21) * kStripS + kStripMaxW + 15 < kStr
 static_assert(kStripS - 15 >= kStripMaxW, "columns -15..-1 must land on zero columns");
 static_assert(kStripGuard + (kStripMaxW + 2 * kStripPad - 1) * kStripS + 128 <= kStripLds,
               "staging of the widest table must stay inside the LDS array");
-static_assert(kTablePadRows >= 7, "direct reads of a touched chunk go 7 rows past a table");
+static_assert(kTablePadRows >= kStripPad && kTablePadRows >= 3,
+              "staging copies the kStripPad zero rows on either side of a table from its padded block; "
+              "direct reads of a touched slot go up to 3 rows past it");
 // Direct read-ahead goes up to 15 rows past a table: the table buffer has
 // kTableGuardRows rows of the widest table before the first and after the
 // last table.
