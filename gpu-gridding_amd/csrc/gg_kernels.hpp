@@ -33,42 +33,51 @@ namespace gg {
 // table's row 0 sits kTablePadRows rows into its block.
 // TI: precision of the caller's kernel stack; TA: precision the gridder
 // accumulates in (the tables are stored in TA; float -> double is exact).
+// One wave per table row (pad rows included): the row's table and row index
+// come from one division per row, and its lanes write the row's W taps in
+// order.  Workgroups stride over a plane's rows (blockIdx.y: plane), so a
+// launch needs few workgroups per plane whatever the planes' sizes.
 template <typename TI, typename TA>
-__global__ void k_expand_tables(const typename V2<TI>::t* __restrict__ conv,
+__global__ __launch_bounds__(256)
+void k_expand_tables(const typename V2<TI>::t* __restrict__ conv,
         int num_planes, int conv_size_half, int oversample, int half_ov,
         const int* __restrict__ support, const unsigned* __restrict__ plane_off,
         typename V2<TA>::t* __restrict__ tables)
 {
     const size_t plane_len = (size_t)conv_size_half * conv_size_half;
     const int noff = half_ov + 1;
+    const int lane = lane_id();
+    const int wave = (int)(threadIdx.x >> 6), waves = (int)(blockDim.x >> 6);
     for (int p = blockIdx.y; p < num_planes; p += gridDim.y)
     {
         const int s = support[p];
         const int W = 2 * s + 1;
-        const unsigned tab_len = (unsigned)(W + 2 * kTablePadRows) * W;
-        const unsigned total = tab_len * noff * noff;
+        const int rows = W + 2 * kTablePadRows;          // rows of one padded table
+        const int total_rows = rows * noff * noff;
         const typename V2<TI>::t* src = conv + plane_len * p;
         typename V2<TA>::t* dst = tables + plane_off[p];
-        for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < total;
-                e += gridDim.x * blockDim.x)
+        for (int rr = blockIdx.x * waves + wave; rr < total_rows; rr += gridDim.x * waves)
         {
-            const unsigned tab = e / tab_len;
-            const unsigned rem = e - tab * tab_len;
-            const int jj = (int)(rem / W) - kTablePadRows;
-            const int kk = (int)(rem % W);
-            typename V2<TA>::t o;
-            o.x = (TA)0;
-            o.y = (TA)0;
-            if (jj >= 0 && jj < W)
+            const int tab = rr / rows;
+            const int jj = rr - tab * rows - kTablePadRows;
+            const int a = tab / noff, b = tab - a * noff;
+            typename V2<TA>::t* out = dst + (size_t)rr * W;
+            const bool in = jj >= 0 && jj < W;
+            const typename V2<TI>::t* srow =
+                    src + (in ? (size_t)abs(b + oversample * (jj - s)) * conv_size_half : 0);
+            for (int kk = lane; kk < W; kk += 64)
             {
-                const int a = (int)tab / noff, b = (int)tab - ((int)tab / noff) * noff;
-                const int row = abs(b + oversample * (jj - s));
-                const int col = abs(a + oversample * (kk - s));
-                const typename V2<TI>::t v = src[(size_t)row * conv_size_half + col];
-                o.x = (TA)v.x;
-                o.y = (TA)v.y;
+                typename V2<TA>::t o;
+                o.x = (TA)0;
+                o.y = (TA)0;
+                if (in)
+                {
+                    const typename V2<TI>::t v = srow[abs(a + oversample * (kk - s))];
+                    o.x = (TA)v.x;
+                    o.y = (TA)v.y;
+                }
+                out[kk] = o;
             }
-            dst[e] = o;
         }
     }
 }
@@ -76,24 +85,26 @@ __global__ void k_expand_tables(const typename V2<TI>::t* __restrict__ conv,
 // norm_table[p][off_v + H][off_u + H] = sum over the footprint of Re(tap) in
 // double: what one visibility with these (plane, off_u, off_v) adds to the
 // norm per unit weight (the `sum` of oskar_grid_wproj.c:154, :207, :210).
-// One wave per entry: lanes take columns, rows in order, then a fixed-order
-// wave reduction (deterministic; agrees with the serial order's double sum
-// to a few ulps).
+// The footprint's taps for -off are those for +off in reverse order (k ->
+// -k), so one wave sums the entry for (|off_u|, |off_v|) -- lanes take
+// columns, rows in order, then a fixed-order wave reduction (deterministic;
+// agrees with the serial order's double sum to a few ulps) -- and writes it
+// to the four entries with those magnitudes.
 template <typename T>
 __global__ __launch_bounds__(256)
 void k_norm_table(const typename V2<T>::t* __restrict__ conv,
         int num_planes, int conv_size_half, int oversample, int half_ov,
         const int* __restrict__ support, double* __restrict__ norm_table)
 {
-    const int side = 2 * half_ov + 1;
-    const int total = num_planes * side * side;
+    const int side = 2 * half_ov + 1, noff = half_ov + 1;
+    const int total = num_planes * noff * noff;
     const int lane = lane_id();
     const int nw = gridDim.x * (blockDim.x >> 6);
     for (int e = blockIdx.x * (blockDim.x >> 6) + (int)(threadIdx.x >> 6); e < total; e += nw)
     {
-        const int p = e / (side * side);
-        const int r = e - p * side * side;
-        const int ov = r / side - half_ov, ou = r % side - half_ov;
+        const int p = e / (noff * noff);
+        const int r = e - p * noff * noff;
+        const int ov = r / noff, ou = r % noff;           // magnitudes
         const int s = support[p];
         const typename V2<T>::t* src = conv + (size_t)conv_size_half * conv_size_half * p;
         double sum = 0.0;
@@ -104,7 +115,11 @@ void k_norm_table(const typename V2<T>::t* __restrict__ conv,
                 sum += (double)row[abs(ou + oversample * k)].x;
         }
         sum = wave_sum(sum);
-        if (lane == 0) norm_table[e] = sum;
+        if (lane < 4)
+        {
+            const int sv = (lane & 2) ? -ov : ov, su = (lane & 1) ? -ou : ou;
+            norm_table[(size_t)p * side * side + (size_t)(sv + half_ov) * side + (su + half_ov)] = sum;
+        }
     }
 }
 

@@ -577,13 +577,14 @@ int prepare_tables(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
                 hipMemcpyHostToDevice, C->stream));
     }
     {
-        const unsigned per_plane = (unsigned)((Wmax + 2 * kTablePadRows) * Wmax * noff * noff);
-        dim3 g(grid_blocks(per_plane, 256, 1024), (unsigned)std::min<size_t>(P, 65535));
+        // four rows (waves) per workgroup, at most 16 workgroups per plane
+        const unsigned rows_max = (unsigned)((Wmax + 2 * kTablePadRows) * noff * noff);
+        dim3 g(grid_blocks(rows_max, 4, 16), (unsigned)std::min<size_t>(P, 65535));
         hipLaunchKernelGGL((k_expand_tables<TI, TA>), g, dim3(256), 0, C->stream,
                 conv_dev, (int)P, csh, ov, half, C->support_d.as<int>(),
                 C->plane_off_d.as<unsigned>(), C->tables.as<T2>());
         HIP_TRY(hipGetLastError());
-        const unsigned entries = (unsigned)(P * (2 * half + 1) * (2 * half + 1));
+        const unsigned entries = (unsigned)(P * noff * noff);   // (|off_u|, |off_v|) per plane
         hipLaunchKernelGGL(k_norm_table<TI>, dim3(grid_blocks(entries, 4, 65535)), dim3(256), 0,
                 C->stream, conv_dev, (int)P, csh, ov, half, C->support_d.as<int>(),
                 C->norm_table.as<double>());
