@@ -260,10 +260,13 @@ void k_select(const SelectArgs<TI, TA> A, unsigned num_chunks)
                 am.y = s.conj ? v_im : -v_im;
                 am.z = v_im;
                 am.w = s.conj ? -v_re : v_re;
-                A.amp[i] = am;
                 amp_re = v_re;
                 amp_im = v_im;
-                if (!isfinite(v_re) || !isfinite(v_im))
+                const bool nonfinite = !isfinite(v_re) || !isfinite(v_im);
+                // (the strip path reads amplitudes from vrec; only the
+                // non-finite pass reads amp there)
+                if (!A.vrec || nonfinite) A.amp[i] = am;
+                if (nonfinite)
                 {
                     flags |= kFlagNonFinite;
                     ++nnonfin;
