@@ -772,7 +772,9 @@ __global__ void k_run_flags(const unsigned* __restrict__ keys,
 
 // Per run r (excl = exclusive scan of flag): run_start[r] = its first visit,
 // run_z[r] / run_tb[r] = packed flags-support-table word and table base of
-// its visits; run_start[R] = n.  The run of visit p is excl[p] + flag[p] - 1.
+// its visits (unless run_z is null: the strip kernel reads the gathered
+// records instead); run_start[R] = n.  The run of visit p is excl[p] +
+// flag[p] - 1.
 __global__ void k_run_starts(const unsigned* __restrict__ flag,
         const unsigned* __restrict__ excl, const unsigned* __restrict__ vals,
         const int4* __restrict__ meta, const unsigned long long* __restrict__ d_n,
@@ -786,10 +788,13 @@ __global__ void k_run_starts(const unsigned* __restrict__ flag,
         if (flag[p])
         {
             const unsigned r = excl[p];
-            const int4 m = meta[vals[p]];
             run_start[r] = (unsigned)p;
-            run_z[r] = m.z;
-            run_tb[r] = m.w;
+            if (run_z)
+            {
+                const int4 m = meta[vals[p]];
+                run_z[r] = m.z;
+                run_tb[r] = m.w;
+            }
         }
         if (p == 0) run_start[*d_runs] = (unsigned)n;
     }

@@ -971,11 +971,15 @@ int run_pipeline(mi355_ctx* C, size_t P, const int* support, int ov, int csh,
         TRY(scan_u32(C, run_flag, run_excl, rc, T_vis, sc + SC_RUNS,
                 reinterpret_cast<unsigned*>(sc + SC_OVERFLOW), 0));
         TRY(ensure(C, C->run_start, (T_vis + 1) * sizeof(unsigned)));
-        TRY(ensure(C, C->run_z, (T_vis + 1) * sizeof(int)));
-        TRY(ensure(C, C->run_tb, (T_vis + 1) * sizeof(int)));
+        if (!strips)   // (the per-run words are for the 64x64 staged kernel)
+        {
+            TRY(ensure(C, C->run_z, (T_vis + 1) * sizeof(int)));
+            TRY(ensure(C, C->run_tb, (T_vis + 1) * sizeof(int)));
+        }
         hipLaunchKernelGGL(k_run_starts, dim3(vb), dim3(256), 0, C->stream, run_flag, run_excl,
                 svals, C->meta.as<int4>(), sc + SC_VISITS, sc + SC_RUNS,
-                C->run_start.as<unsigned>(), C->run_z.as<int>(), C->run_tb.as<int>());
+                C->run_start.as<unsigned>(), strips ? nullptr : C->run_z.as<int>(),
+                strips ? nullptr : C->run_tb.as<int>());
         HIP_TRY(hipGetLastError());
     }
     if (strips)
