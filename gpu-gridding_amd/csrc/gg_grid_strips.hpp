@@ -10,29 +10,33 @@
 //
 // The region's visits are sorted by kernel table, so its list is a sequence
 // of runs that read one table (in any of the four mirror orientations).  A
-// run of at least min_stage_run visits whose table is at most kStripMaxW
-// wide is staged in LDS once, unmirrored, at a fixed row stride with zero
-// rows and columns around it; every visit of the run then reads a lane's tap
-// of slot i at a constant offset from one per-visit base address.  Other
-// visits read taps straight from the padded tables in global memory with
-// raw buffer loads.
+// run at least as long as its table-width class's threshold, whose table is
+// at most kStripMaxW wide, is staged in LDS once, unmirrored, at a fixed row
+// stride with zero rows and columns around it; every visit of the run then
+// reads a lane's tap of slot i at a constant offset from one per-visit base
+// address.  Other visits read taps straight from the padded tables in global
+// memory with raw buffer loads.
 //
 // Rows go in chunks of eight (two slots).  A visit touches chunks q0..q1 of
 // the strip.  From LDS it takes the pairs of chunks (16 rows, four slots)
 // that hold them, the four slots of a pair read together and all four
-// multiplied; from global memory the touched chunks are requested
-// kDirectGroup at a time, others skipped with a wave-uniform branch.  Rows
-// outside the footprint read zero rows, columns outside it zero columns
-// (staged) or nothing (direct: the buffer offset is out of range).
+// multiplied; from global memory the touched chunks' footprint slots are
+// requested kDirectGroup chunks at a time, others skipped with wave-uniform
+// branches.  Rows outside the footprint read zero rows, columns outside it
+// zero columns (staged) or nothing (direct: the buffer offset is out of
+// range).
 //
-// Each wave walks all visits of a run segment 64 at a time: one vector load
-// per lane of the records, a ballot of the visits whose footprint meets its
-// strip, then those visits one by one with their records broadcast by
-// readlane and the per-visit addressing done in scalar registers.  Every
-// load in the loop nest is a raw buffer load (scalar base, 32-bit offset),
-// which keeps per-lane address registers to one per stream.  At the end of
-// a work item the strip is added to the grid (single-item regions) or stored
-// in the item's partial slot (merged in item order, FP64, afterwards).
+// A work item's direct visits go first (pass 1; barriers keep the waves on
+// the same visits where they read wide tables, so that neighbouring strips
+// share the lines they fetch), then its staged runs (pass 2).  Each wave
+// walks all visits of a run segment 64 at a time: one vector load per lane
+// of the records, a ballot of the visits whose footprint meets its strip,
+// then those visits one by one with their records broadcast by readlane and
+// the per-visit addressing done in scalar registers.  Every load in the loop
+// nest is a raw buffer load (scalar base, 32-bit offset), which keeps
+// per-lane address registers to one per stream.  At the end of a work item
+// the strip is added to the grid (single-item regions) or stored in the
+// item's partial slot (merged in item order, FP64, afterwards).
 #pragma once
 
 #include "gg_kernels.hpp"
